@@ -1,0 +1,636 @@
+/*
+ * mg_oracle.c -- CPU FP64 restatement of MultiFlow3D's pressure multigrid.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library and the CPU baseline timed beside it; the product never links it.
+ *
+ * It restates, in its own data layout (one ghosted array per block and level,
+ * i fastest), the arithmetic and the order of operations of:
+ *   coeff    mgsolver.for:2-222     coefficient hierarchy (constant and LSM)
+ *   mgkcyc   mgsolver.for:224-369   V-cycle driven by the K-cycle counter
+ *   mgrelax  mgsolver.for:547-611   red-black Gauss-Seidel (mg_itrsch = 3)
+ *   mgrestr  mgsolver.for:619-761   residual + 8-cell average restriction
+ *   mgcorr   mgsolver.for:763-905   trilinear prolongation + correction
+ *   mgbound  mgsolver.for:907-992   Neumann ghost copy
+ *   exchangepp / exchbc_mgpp        exchangepp.for:43-430,3997-4763,4768-5383
+ *                                   (same-resolution, same-rank branches)
+ * Every floating-point expression keeps the reference's operand order
+ * (Fortran evaluates a+b+c left to right; the reference is built without
+ * FMA contraction), so results are bitwise comparable with the compiled
+ * reference.  Build with -ffp-contract=off.
+ *
+ * Scope notes (what is not restated):
+ *  - smoothers 1 and 2 (lexicographic GS, ADI-TDMA): mgsolver.for:403-545;
+ *  - refined-block (rdv unequal) exchange branches and the injection branch
+ *    for blocks with fewer levels (mgsolver.for:719-732, 807-819);
+ *  - the explicit corner/edge messages of exchangepp.for:433-1093: on a
+ *    conforming block grid they carry the same values the three full-face
+ *    passes already deliver (each face pass copies whole ghosted planes,
+ *    so x -> y -> z propagates edges and corners).  Multi-block fixtures
+ *    from the compiled reference pin this.
+ */
+#include "mg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { FW = 0, FE = 1, FS = 2, FN = 3, FB = 4, FT = 5 };
+
+struct orc_mg {
+    int nb, L, lsm, nu_pre, nu_post, periodic;
+    orc_block *blk;
+    int32_t (*dims)[3]; /* [b*L + l] interior dims */
+    double **phi;       /* [b*L + l] ghosted */
+    double **rhs;       /* [b*L + l] interior */
+    double **cof;       /* [b*L + l] 7 interior arrays: aw ae as an ab at ap */
+    double **dens;      /* [b*L + l] interior (lsm only) */
+    double **bufm;      /* [b] staging for the minus-side ghost plane */
+    double **bufp;      /* [b] staging for the plus-side ghost plane */
+    double **pf;        /* [b] ghosted level-0 pressure used by orc_solve */
+};
+
+static inline size_t gsz(const int32_t *n) {
+    return (size_t)(n[0] + 2) * (size_t)(n[1] + 2) * (size_t)(n[2] + 2);
+}
+static inline size_t isz(const int32_t *n) {
+    return (size_t)n[0] * (size_t)n[1] * (size_t)n[2];
+}
+/* ghosted index, coordinates 0..n+1 */
+static inline size_t gi(const int32_t *n, int i, int j, int k) {
+    return (size_t)i + (size_t)(n[0] + 2) * ((size_t)j + (size_t)(n[1] + 2) * (size_t)k);
+}
+/* interior index, coordinates 1..n */
+static inline size_t ci(const int32_t *n, int i, int j, int k) {
+    return (size_t)(i - 1) + (size_t)n[0] * ((size_t)(j - 1) + (size_t)n[1] * (size_t)(k - 1));
+}
+
+#define DIMS(mg, b, l) ((mg)->dims[(size_t)(b) * (mg)->L + (l)])
+#define SLOT(mg, b, l) ((size_t)(b) * (mg)->L + (l))
+
+orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
+                   int nu_pre, int nu_post) {
+    if (nblocks <= 0 || levels <= 0) return NULL;
+    for (int b = 0; b < nblocks; ++b)
+        for (int d = 0; d < 3; ++d) {
+            int n = blocks[b].n[d];
+            if (n <= 0 || (n % (1 << (levels - 1))) != 0) return NULL;
+        }
+    orc_mg *mg = (orc_mg *)calloc(1, sizeof(orc_mg));
+    mg->nb = nblocks;
+    mg->L = levels;
+    mg->lsm = lsm;
+    mg->nu_pre = nu_pre;
+    mg->nu_post = nu_post;
+    mg->blk = (orc_block *)malloc(sizeof(orc_block) * nblocks);
+    memcpy(mg->blk, blocks, sizeof(orc_block) * nblocks);
+    size_t ns = (size_t)nblocks * levels;
+    mg->dims = malloc(sizeof(*mg->dims) * ns);
+    mg->phi = calloc(ns, sizeof(double *));
+    mg->rhs = calloc(ns, sizeof(double *));
+    mg->cof = calloc(ns, sizeof(double *));
+    mg->dens = calloc(ns, sizeof(double *));
+    mg->bufm = calloc(nblocks, sizeof(double *));
+    mg->bufp = calloc(nblocks, sizeof(double *));
+    mg->pf = calloc(nblocks, sizeof(double *));
+    mg->periodic = 0;
+    size_t face_max = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        for (int f = 0; f < 6; ++f)
+            if (blocks[b].bc[f] == 5) mg->periodic = 1; /* alloc_dom.for:1156-1163 */
+        for (int l = 0; l < levels; ++l) {
+            int32_t *n = DIMS(mg, b, l);
+            for (int d = 0; d < 3; ++d) n[d] = blocks[b].n[d] >> l;
+            mg->phi[SLOT(mg, b, l)] = calloc(gsz(n), sizeof(double));
+            mg->rhs[SLOT(mg, b, l)] = calloc(isz(n), sizeof(double));
+            mg->cof[SLOT(mg, b, l)] = calloc(7 * isz(n), sizeof(double));
+            if (lsm) mg->dens[SLOT(mg, b, l)] = calloc(isz(n), sizeof(double));
+        }
+        const int32_t *n0 = DIMS(mg, b, 0);
+        size_t f01 = (size_t)(n0[0] + 2) * (n0[1] + 2), f02 = (size_t)(n0[0] + 2) * (n0[2] + 2),
+               f12 = (size_t)(n0[1] + 2) * (n0[2] + 2);
+        size_t fm = f01 > f02 ? f01 : f02;
+        fm = fm > f12 ? fm : f12;
+        if (fm > face_max) face_max = fm;
+        mg->pf[b] = calloc(gsz(n0), sizeof(double));
+    }
+    for (int b = 0; b < nblocks; ++b) {
+        mg->bufm[b] = calloc(face_max, sizeof(double));
+        mg->bufp[b] = calloc(face_max, sizeof(double));
+    }
+    return mg;
+}
+
+void orc_destroy(orc_mg *mg) {
+    if (!mg) return;
+    size_t ns = (size_t)mg->nb * mg->L;
+    for (size_t s = 0; s < ns; ++s) {
+        free(mg->phi[s]);
+        free(mg->rhs[s]);
+        free(mg->cof[s]);
+        free(mg->dens[s]);
+    }
+    for (int b = 0; b < mg->nb; ++b) {
+        free(mg->bufm[b]);
+        free(mg->bufp[b]);
+        free(mg->pf[b]);
+    }
+    free(mg->phi); free(mg->rhs); free(mg->cof); free(mg->dens);
+    free(mg->bufm); free(mg->bufp); free(mg->pf);
+    free(mg->dims); free(mg->blk); free(mg);
+}
+
+int orc_dims(const orc_mg *mg, int block, int level, int32_t out_n[3]) {
+    if (block < 0 || block >= mg->nb || level < 0 || level >= mg->L) return -1;
+    const int32_t *n = mg->dims[(size_t)block * mg->L + level];
+    out_n[0] = n[0]; out_n[1] = n[1]; out_n[2] = n[2];
+    return 0;
+}
+double *orc_phi(orc_mg *mg, int b, int l) { return mg->phi[SLOT(mg, b, l)]; }
+double *orc_rhs(orc_mg *mg, int b, int l) { return mg->rhs[SLOT(mg, b, l)]; }
+double *orc_cof(orc_mg *mg, int b, int l, int which) {
+    return mg->cof[SLOT(mg, b, l)] + (size_t)which * isz(DIMS(mg, b, l));
+}
+double *orc_dens(orc_mg *mg, int b, int l) { return mg->dens[SLOT(mg, b, l)]; }
+
+int orc_set_density(orc_mg *mg, const double *dens) {
+    if (!mg->lsm) return -1;
+    size_t off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        size_t n = isz(DIMS(mg, b, 0));
+        memcpy(mg->dens[SLOT(mg, b, 0)], dens + off, n * sizeof(double));
+        off += n;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* coeff (mgsolver.for:2-222)                                               */
+/* ------------------------------------------------------------------------ */
+void orc_coeff(orc_mg *mg) {
+    for (int b = 0; b < mg->nb; ++b) {
+        const orc_block *bk = &mg->blk[b];
+        for (int l = 0; l < mg->L; ++l) {
+            const int32_t *n = DIMS(mg, b, l);
+            size_t N = isz(n);
+            memset(mg->phi[SLOT(mg, b, l)], 0, gsz(n) * sizeof(double)); /* :60-67 */
+            double *c = mg->cof[SLOT(mg, b, l)];
+            double *aw = c, *ae = c + N, *as = c + 2 * N, *an = c + 3 * N, *ab = c + 4 * N,
+                   *at = c + 5 * N, *ap = c + 6 * N;
+            double *rh = mg->rhs[SLOT(mg, b, l)];
+            double *dl = mg->lsm ? mg->dens[SLOT(mg, b, l)] : NULL;
+            if (mg->lsm && l > 0) { /* :81-102, 8-cell average in the ij1..ij8 order */
+                const int32_t *nf = DIMS(mg, b, l - 1);
+                const double *df = mg->dens[SLOT(mg, b, l - 1)];
+                for (int k = 1; k <= n[2]; ++k)
+                    for (int j = 1; j <= n[1]; ++j)
+                        for (int i = 1; i <= n[0]; ++i) {
+                            int fi = 2 * i - 1, fj = 2 * j - 1, fk = 2 * k - 1;
+                            double s = df[ci(nf, fi, fj, fk)];
+                            s = s + df[ci(nf, fi, fj + 1, fk)];
+                            s = s + df[ci(nf, fi + 1, fj, fk)];
+                            s = s + df[ci(nf, fi + 1, fj + 1, fk)];
+                            s = s + df[ci(nf, fi, fj, fk + 1)];
+                            s = s + df[ci(nf, fi, fj + 1, fk + 1)];
+                            s = s + df[ci(nf, fi + 1, fj, fk + 1)];
+                            s = s + df[ci(nf, fi + 1, fj + 1, fk + 1)];
+                            dl[ci(n, i, j, k)] = 0.125 * s;
+                        }
+            }
+            /* constant branch :184-189: 1/h_l^2 with h_l = h * 2^l */
+            double ndx = bk->h[0] * (double)(1 << l), ndy = bk->h[1] * (double)(1 << l),
+                   ndz = bk->h[2] * (double)(1 << l);
+            double cxx = 1.0 / (ndx * ndx), cyy = 1.0 / (ndy * ndy), czz = 1.0 / (ndz * ndz);
+            /* LSM branch :170-181 uses the finest dx, dy, dz on every level */
+            double gx = bk->h[1] * bk->h[2] / bk->h[0];
+            double gy = bk->h[0] * bk->h[2] / bk->h[1];
+            double gz = bk->h[0] * bk->h[1] / bk->h[2];
+            for (int k = 1; k <= n[2]; ++k)
+                for (int j = 1; j <= n[1]; ++j)
+                    for (int i = 1; i <= n[0]; ++i) {
+                        size_t q = ci(n, i, j, k);
+                        if (mg->lsm) {
+                            double d0 = dl[q];
+                            double dim = (i == 1) ? d0 : 0.5 * (d0 + dl[q - 1]);
+                            double dip = (i == n[0]) ? d0 : 0.5 * (d0 + dl[q + 1]);
+                            double djm = (j == 1) ? d0 : 0.5 * (d0 + dl[q - n[0]]);
+                            double djp = (j == n[1]) ? d0 : 0.5 * (d0 + dl[q + n[0]]);
+                            size_t sk = (size_t)n[0] * n[1];
+                            double dkm = (k == 1) ? d0 : 0.5 * (d0 + dl[q - sk]);
+                            double dkp = (k == n[2]) ? d0 : 0.5 * (d0 + dl[q + sk]);
+                            aw[q] = gx / dim;
+                            ae[q] = gx / dip;
+                            as[q] = gy / djm;
+                            an[q] = gy / djp;
+                            ab[q] = gz / dkm;
+                            at[q] = gz / dkp;
+                        } else {
+                            aw[q] = cxx; ae[q] = cxx;
+                            as[q] = cyy; an[q] = cyy;
+                            ab[q] = czz; at[q] = czz;
+                        }
+                        /* :192-203: zero the coefficient on a physical non-periodic face */
+                        if (bk->nbr[FW] < 0 && bk->bc[FW] != 5 && i == 1) aw[q] = 0.0;
+                        if (bk->nbr[FE] < 0 && bk->bc[FE] != 5 && i == n[0]) ae[q] = 0.0;
+                        if (bk->nbr[FS] < 0 && bk->bc[FS] != 5 && j == 1) as[q] = 0.0;
+                        if (bk->nbr[FN] < 0 && bk->bc[FN] != 5 && j == n[1]) an[q] = 0.0;
+                        if (bk->nbr[FB] < 0 && bk->bc[FB] != 5 && k == 1) ab[q] = 0.0;
+                        if (bk->nbr[FT] < 0 && bk->bc[FT] != 5 && k == n[2]) at[q] = 0.0;
+                        /* :205-208 */
+                        ap[q] = -1.0 * (aw[q] + ae[q] + as[q] + an[q] + ab[q] + at[q]);
+                        rh[q] = 0.0; /* :210 */
+                    }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* ghost planes                                                             */
+/* ------------------------------------------------------------------------ */
+/* Copy the full ghosted plane coord_d == c of a ghosted array to/from buf. */
+static void plane_get(const double *a, const int32_t *n, int d, int c, double *buf) {
+    size_t m = 0;
+    if (d == 0) {
+        for (int k = 0; k <= n[2] + 1; ++k)
+            for (int j = 0; j <= n[1] + 1; ++j) buf[m++] = a[gi(n, c, j, k)];
+    } else if (d == 1) {
+        for (int k = 0; k <= n[2] + 1; ++k)
+            for (int i = 0; i <= n[0] + 1; ++i) buf[m++] = a[gi(n, i, c, k)];
+    } else {
+        for (int j = 0; j <= n[1] + 1; ++j)
+            for (int i = 0; i <= n[0] + 1; ++i) buf[m++] = a[gi(n, i, j, c)];
+    }
+}
+static void plane_set(double *a, const int32_t *n, int d, int c, const double *buf) {
+    size_t m = 0;
+    if (d == 0) {
+        for (int k = 0; k <= n[2] + 1; ++k)
+            for (int j = 0; j <= n[1] + 1; ++j) a[gi(n, c, j, k)] = buf[m++];
+    } else if (d == 1) {
+        for (int k = 0; k <= n[2] + 1; ++k)
+            for (int i = 0; i <= n[0] + 1; ++i) a[gi(n, i, c, k)] = buf[m++];
+    } else {
+        for (int j = 0; j <= n[1] + 1; ++j)
+            for (int i = 0; i <= n[0] + 1; ++i) a[gi(n, i, j, c)] = buf[m++];
+    }
+}
+static void plane_copy(double *a, const int32_t *n, int d, int dst, int src) {
+    if (d == 0) {
+        for (int k = 0; k <= n[2] + 1; ++k)
+            for (int j = 0; j <= n[1] + 1; ++j) a[gi(n, dst, j, k)] = a[gi(n, src, j, k)];
+    } else if (d == 1) {
+        for (int k = 0; k <= n[2] + 1; ++k)
+            for (int i = 0; i <= n[0] + 1; ++i) a[gi(n, i, dst, k)] = a[gi(n, i, src, k)];
+    } else {
+        for (int j = 0; j <= n[1] + 1; ++j)
+            for (int i = 0; i <= n[0] + 1; ++i) a[gi(n, i, j, dst)] = a[gi(n, i, j, src)];
+    }
+}
+
+/* mgbound (mgsolver.for:907-992): Neumann copy, faces in order W,E,S,N,B,T. */
+static void bound_fields(orc_mg *mg, int l, double **f) {
+    for (int b = 0; b < mg->nb; ++b) {
+        const orc_block *bk = &mg->blk[b];
+        const int32_t *n = DIMS(mg, b, l);
+        for (int d = 0; d < 3; ++d) {
+            if (bk->nbr[2 * d] < 0 && bk->bc[2 * d] != 5) plane_copy(f[b], n, d, 0, 1);
+            if (bk->nbr[2 * d + 1] < 0 && bk->bc[2 * d + 1] != 5)
+                plane_copy(f[b], n, d, n[d] + 1, n[d]);
+        }
+    }
+}
+
+/* exchangepp (same-rank, same-resolution) followed by mgbound and
+ * exchbc_mgpp; f[b] is block b's ghosted array at level l. */
+static void exchange_fields(orc_mg *mg, int l, double **f) {
+    /* face passes x, y, z: all blocks pack, then all blocks unpack
+     * (exchangepp.for:43-430 pack, 3997-4127 unpack) */
+    for (int d = 0; d < 3; ++d) {
+        for (int b = 0; b < mg->nb; ++b) {
+            const orc_block *bk = &mg->blk[b];
+            const int32_t *n = DIMS(mg, b, l);
+            int prv = bk->nbr[2 * d], nxt = bk->nbr[2 * d + 1];
+            if (prv >= 0) plane_get(f[b], n, d, 1, mg->bufp[prv]);
+            if (nxt >= 0) plane_get(f[b], n, d, n[d], mg->bufm[nxt]);
+        }
+        for (int b = 0; b < mg->nb; ++b) {
+            const orc_block *bk = &mg->blk[b];
+            const int32_t *n = DIMS(mg, b, l);
+            if (bk->nbr[2 * d] >= 0) plane_set(f[b], n, d, 0, mg->bufm[b]);
+            if (bk->nbr[2 * d + 1] >= 0) plane_set(f[b], n, d, n[d] + 1, mg->bufp[b]);
+        }
+    }
+    bound_fields(mg, l, f); /* exchangepp.for:4761 */
+    if (!mg->periodic) return;
+    /* exchbc_mgpp (exchangepp.for:4768-5383), directions x, y, z */
+    for (int d = 0; d < 3; ++d) {
+        for (int b = 0; b < mg->nb; ++b) {
+            const orc_block *bk = &mg->blk[b];
+            if (!(bk->bc[2 * d] == 5 || bk->bc[2 * d + 1] == 5)) continue;
+            const int32_t *n = DIMS(mg, b, l);
+            int prv = bk->nbr[2 * d], nxt = bk->nbr[2 * d + 1];
+            if (prv < 0) {
+                if (nxt < 0) plane_copy(f[b], n, d, 0, n[d]);
+                else if (bk->per[2 * d] >= 0) plane_get(f[b], n, d, 1, mg->bufp[bk->per[2 * d]]);
+            }
+            if (nxt < 0) {
+                if (prv < 0) plane_copy(f[b], n, d, n[d] + 1, 1);
+                else if (bk->per[2 * d + 1] >= 0)
+                    plane_get(f[b], n, d, n[d], mg->bufm[bk->per[2 * d + 1]]);
+            }
+        }
+        for (int b = 0; b < mg->nb; ++b) {
+            const orc_block *bk = &mg->blk[b];
+            if (!(bk->bc[2 * d] == 5 || bk->bc[2 * d + 1] == 5)) continue;
+            const int32_t *n = DIMS(mg, b, l);
+            int prv = bk->nbr[2 * d], nxt = bk->nbr[2 * d + 1];
+            if (prv < 0 && nxt >= 0 && bk->per[2 * d] >= 0) plane_set(f[b], n, d, 0, mg->bufm[b]);
+            if (nxt < 0 && prv >= 0 && bk->per[2 * d + 1] >= 0)
+                plane_set(f[b], n, d, n[d] + 1, mg->bufp[b]);
+        }
+    }
+}
+
+static double **level_phis(orc_mg *mg, int l, double **tmp) {
+    for (int b = 0; b < mg->nb; ++b) tmp[b] = mg->phi[SLOT(mg, b, l)];
+    return tmp;
+}
+
+void orc_exchange(orc_mg *mg, int l) {
+    double **f = malloc(sizeof(double *) * mg->nb);
+    exchange_fields(mg, l, level_phis(mg, l, f));
+    free(f);
+}
+
+void orc_bound(orc_mg *mg, int l) {
+    double **f = malloc(sizeof(double *) * mg->nb);
+    bound_fields(mg, l, level_phis(mg, l, f));
+    free(f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* mgrelax, mg_itrsch = 3 (mgsolver.for:547-611)                            */
+/* ------------------------------------------------------------------------ */
+static void rb_colour(orc_mg *mg, int b, int l, int parity) {
+    const int32_t *n = DIMS(mg, b, l);
+    size_t N = isz(n);
+    double *p = mg->phi[SLOT(mg, b, l)];
+    const double *c = mg->cof[SLOT(mg, b, l)];
+    const double *rh = mg->rhs[SLOT(mg, b, l)];
+    size_t sy = (size_t)(n[0] + 2), sz = sy * (size_t)(n[1] + 2);
+    for (int k = 1; k <= n[2]; ++k)
+        for (int j = 1; j <= n[1]; ++j)
+            for (int i = 1; i <= n[0]; ++i) {
+                /* first colour: mod(i+j+k,2)==0 on the reference's 1-based
+                 * ghosted indices == (i+j+k) odd on interior indices here */
+                if (((i + j + k) & 1) != parity) continue;
+                size_t q = ci(n, i, j, k), g = gi(n, i, j, k);
+                double s = c[q] * p[g - 1];
+                s = s + c[N + q] * p[g + 1];
+                s = s + c[2 * N + q] * p[g - sy];
+                s = s + c[3 * N + q] * p[g + sy];
+                s = s + c[4 * N + q] * p[g - sz];
+                s = s + c[5 * N + q] * p[g + sz];
+                p[g] = (rh[q] - s) / c[6 * N + q];
+            }
+}
+
+void orc_relax(orc_mg *mg, int l, int sweeps) {
+    double **f = malloc(sizeof(double *) * mg->nb);
+    level_phis(mg, l, f);
+    for (int it = 0; it < sweeps; ++it) {
+        for (int b = 0; b < mg->nb; ++b) {
+            rb_colour(mg, b, l, 1);
+            rb_colour(mg, b, l, 0);
+        }
+        bound_fields(mg, l, f);   /* mgsolver.for:610 */
+        exchange_fields(mg, l, f); /* mgsolver.for:611 */
+    }
+    free(f);
+}
+
+/* r = rhs - (aw W + ae E + as S + an N + ab B + at T + ap P)  (:702-709) */
+static void residual_block(orc_mg *mg, int b, int l, const double *p, const double *rh, double *r) {
+    const int32_t *n = DIMS(mg, b, l);
+    size_t N = isz(n);
+    const double *c = mg->cof[SLOT(mg, b, l)];
+    size_t sy = (size_t)(n[0] + 2), sz = sy * (size_t)(n[1] + 2);
+    for (int k = 1; k <= n[2]; ++k)
+        for (int j = 1; j <= n[1]; ++j)
+            for (int i = 1; i <= n[0]; ++i) {
+                size_t q = ci(n, i, j, k), g = gi(n, i, j, k);
+                double s = c[q] * p[g - 1];
+                s = s + c[N + q] * p[g + 1];
+                s = s + c[2 * N + q] * p[g - sy];
+                s = s + c[3 * N + q] * p[g + sy];
+                s = s + c[4 * N + q] * p[g - sz];
+                s = s + c[5 * N + q] * p[g + sz];
+                s = s + c[6 * N + q] * p[g];
+                r[q] = rh[q] - s;
+            }
+}
+
+void orc_residual(orc_mg *mg, int l, double *r) {
+    size_t off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        residual_block(mg, b, l, mg->phi[SLOT(mg, b, l)], mg->rhs[SLOT(mg, b, l)], r + off);
+        off += isz(DIMS(mg, b, l));
+    }
+}
+
+/* mgrestr (mgsolver.for:619-761) */
+void orc_restrict(orc_mg *mg, int lc) {
+    int lf = lc - 1;
+    for (int b = 0; b < mg->nb; ++b) {
+        const int32_t *nc = DIMS(mg, b, lc), *nf = DIMS(mg, b, lf);
+        memset(mg->phi[SLOT(mg, b, lc)], 0, gsz(nc) * sizeof(double)); /* :650-657 */
+        double *rc = mg->rhs[SLOT(mg, b, lc)];
+        memset(rc, 0, isz(nc) * sizeof(double)); /* :659-667 */
+        double *resf = malloc(isz(nf) * sizeof(double));
+        residual_block(mg, b, lf, mg->phi[SLOT(mg, b, lf)], mg->rhs[SLOT(mg, b, lf)], resf);
+        for (int k = 1; k <= nc[2]; ++k)
+            for (int j = 1; j <= nc[1]; ++j)
+                for (int i = 1; i <= nc[0]; ++i) {
+                    int x0 = 2 * i - 1, y0 = 2 * j - 1, z0 = 2 * k - 1;
+                    double s = resf[ci(nf, x0, y0, z0)]; /* order of :743-747 */
+                    s = s + resf[ci(nf, x0 + 1, y0, z0)];
+                    s = s + resf[ci(nf, x0, y0 + 1, z0)];
+                    s = s + resf[ci(nf, x0 + 1, y0 + 1, z0)];
+                    s = s + resf[ci(nf, x0, y0, z0 + 1)];
+                    s = s + resf[ci(nf, x0 + 1, y0, z0 + 1)];
+                    s = s + resf[ci(nf, x0, y0 + 1, z0 + 1)];
+                    s = s + resf[ci(nf, x0 + 1, y0 + 1, z0 + 1)];
+                    rc[ci(nc, i, j, k)] = 0.125 * s;
+                }
+        free(resf);
+    }
+}
+
+/* mgcorr (mgsolver.for:763-905): cell-centred trilinear prolongation added
+ * to every fine point including the ghost layer, then mgbound only. */
+void orc_correct(orc_mg *mg, int lf) {
+    int lc = lf + 1;
+    /* coarse stencil terms T0..T7 as offsets (di,dj,dk), in the reference's
+     * summation order: c, c+j, c+k, c+jk, c+i, c+ij, c+ik, c+ijk */
+    static const int toff[8][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1},
+                                   {1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
+    for (int b = 0; b < mg->nb; ++b) {
+        const int32_t *nc = DIMS(mg, b, lc), *nf = DIMS(mg, b, lf);
+        const double *pc = mg->phi[SLOT(mg, b, lc)];
+        double *pfi = mg->phi[SLOT(mg, b, lf)];
+        for (int kc = 0; kc <= nc[2]; ++kc)
+            for (int ic = 0; ic <= nc[0]; ++ic)
+                for (int jc = 0; jc <= nc[1]; ++jc) {
+                    double cv[8];
+                    for (int t = 0; t < 8; ++t)
+                        cv[t] = pc[gi(nc, ic + toff[t][0], jc + toff[t][1], kc + toff[t][2])];
+                    for (int ch = 0; ch < 8; ++ch) {
+                        int fi = (ch >> 2) & 1, fj = (ch >> 1) & 1, fk = ch & 1;
+                        double s = 0.0;
+                        for (int t = 0; t < 8; ++t) {
+                            double w = (toff[t][0] == fi ? 3.0 : 1.0) * (toff[t][1] == fj ? 3.0 : 1.0) *
+                                       (toff[t][2] == fk ? 3.0 : 1.0);
+                            s = (t == 0) ? w * cv[0] : s + w * cv[t];
+                        }
+                        size_t g = gi(nf, 2 * ic + fi, 2 * jc + fj, 2 * kc + fk);
+                        pfi[g] = pfi[g] + s / 64.0;
+                    }
+                }
+        (void)nf;
+    }
+    orc_bound(mg, lf); /* mgsolver.for:902 */
+}
+
+/* ------------------------------------------------------------------------ */
+/* mgkcyc (mgsolver.for:224-369), kcycle = 1                                */
+/* ------------------------------------------------------------------------ */
+void orc_vcycle(orc_mg *mg, const double *rhs, double *pp) {
+    size_t oi = 0, og = 0;
+    for (int b = 0; b < mg->nb; ++b) { /* :236-265 load */
+        const int32_t *n = DIMS(mg, b, 0);
+        memcpy(mg->phi[SLOT(mg, b, 0)], pp + og, gsz(n) * sizeof(double));
+        memcpy(mg->rhs[SLOT(mg, b, 0)], rhs + oi, isz(n) * sizeof(double));
+        oi += isz(n);
+        og += gsz(n);
+    }
+    const int L = mg->L;
+    orc_relax(mg, 0, mg->nu_pre); /* :273 */
+    if (L > 1) {
+        orc_restrict(mg, 1); /* :281 */
+        int *kount = calloc(L, sizeof(int));
+        int l = 1, nrel = mg->nu_pre;
+        while (l != 0) { /* label 10 */
+            kount[l] += 1;
+            orc_relax(mg, l, nrel);
+            if (kount[l] == 2) {
+                orc_correct(mg, l - 1);
+                kount[l] = 0;
+                l -= 1;
+                nrel = mg->nu_post;
+            } else if (l < L - 1) {
+                orc_restrict(mg, l + 1);
+                l += 1;
+                nrel = mg->nu_pre;
+            } else {
+                orc_relax(mg, l, mg->nu_post); /* :332 */
+                orc_correct(mg, l - 1);
+                nrel = mg->nu_post;
+                l = L - 2;
+            }
+        }
+        free(kount);
+    }
+    orc_relax(mg, 0, mg->nu_post); /* :349 */
+    og = 0;
+    for (int b = 0; b < mg->nb; ++b) { /* :351-364 store */
+        const int32_t *n = DIMS(mg, b, 0);
+        memcpy(pp + og, mg->phi[SLOT(mg, b, 0)], gsz(n) * sizeof(double));
+        og += gsz(n);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Poisson-form outer loop (newsolv_mg.for:19-58 with calvel+calmas replaced
+ * by the identity div(u* - dt grad p)/dt = f - A p)                          */
+/* ------------------------------------------------------------------------ */
+static void norms(orc_mg *mg, const double *r, double *l2, double *rmax) {
+    double s = 0.0, m = 0.0;
+    size_t off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        const int32_t *n = DIMS(mg, b, 0);
+        const orc_block *bk = &mg->blk[b];
+        size_t N = isz(n);
+        for (size_t q = 0; q < N; ++q) {
+            double v = r[off + q];
+            s = s + v * v;
+            /* press.for:60-62: rmax = max |div * dx*dy*dz|; the LSM RHS
+             * already carries the cell volume (press.for:51-53) */
+            double a = mg->lsm ? fabs(v) : fabs(v * bk->h[0] * bk->h[1] * bk->h[2]);
+            if (a > m) m = a;
+        }
+        off += N;
+    }
+    *l2 = sqrt(s);
+    *rmax = m;
+}
+
+int orc_solve(orc_mg *mg, const double *f, double *p, double tol, int maxcy, int min_cycles,
+              double *hist_rho, double *hist_rmax) {
+    size_t ntot = 0, gtot = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        ntot += isz(DIMS(mg, b, 0));
+        gtot += gsz(DIMS(mg, b, 0));
+    }
+    double *r = malloc(ntot * sizeof(double));
+    double *pp = malloc(gtot * sizeof(double));
+    for (int b = 0; b < mg->nb; ++b) memset(mg->pf[b], 0, gsz(DIMS(mg, b, 0)) * sizeof(double));
+    size_t off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        residual_block(mg, b, 0, mg->pf[b], f + off, r + off);
+        off += isz(DIMS(mg, b, 0));
+    }
+    double nf, rm, nr;
+    norms(mg, f, &nf, &rm);
+    norms(mg, r, &nr, &rm);
+    if (nf == 0.0) nf = 1.0;
+    hist_rho[0] = nr / nf;
+    hist_rmax[0] = rm;
+    int k = 0;
+    for (k = 1; k <= maxcy; ++k) {
+        memset(pp, 0, gtot * sizeof(double)); /* newsolv_mg.for:19-21,44 */
+        orc_vcycle(mg, r, pp);
+        size_t og = 0;
+        for (int b = 0; b < mg->nb; ++b) { /* newsolv_mg.for:38-48, ppref == 0 */
+            const int32_t *n = DIMS(mg, b, 0);
+            for (int kk = 1; kk <= n[2]; ++kk)
+                for (int j = 1; j <= n[1]; ++j)
+                    for (int i = 1; i <= n[0]; ++i) {
+                        size_t g = gi(n, i, j, kk);
+                        mg->pf[b][g] = mg->pf[b][g] + pp[og + g];
+                    }
+            og += gsz(n);
+        }
+        exchange_fields(mg, 0, mg->pf); /* exchange(4) + pbound */
+        off = 0;
+        for (int b = 0; b < mg->nb; ++b) {
+            residual_block(mg, b, 0, mg->pf[b], f + off, r + off);
+            off += isz(DIMS(mg, b, 0));
+        }
+        norms(mg, r, &nr, &rm);
+        hist_rho[k] = nr / nf;
+        hist_rmax[k] = rm;
+        if (hist_rho[k] <= tol && k >= min_cycles) break;
+    }
+    if (k > maxcy) k = maxcy;
+    off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        const int32_t *n = DIMS(mg, b, 0);
+        for (int kk = 1; kk <= n[2]; ++kk)
+            for (int j = 1; j <= n[1]; ++j)
+                for (int i = 1; i <= n[0]; ++i) p[off + ci(n, i, j, kk)] = mg->pf[b][gi(n, i, j, kk)];
+        off += isz(n);
+    }
+    free(r);
+    free(pp);
+    return k;
+}

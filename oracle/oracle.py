@@ -1,0 +1,235 @@
+"""ctypes front end of the CPU oracle (liborc_mg.so) and of the compiled
+reference driver (_ref/mg_ref).
+
+TEST INFRASTRUCTURE ONLY.  Import it from tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg, and nowhere else: it is the checker, never
+the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+from typing import List, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc_mg.so")
+REF_BIN = os.path.join(HERE, "_ref", "mg_ref")
+
+
+class OrcBlock(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("h", ctypes.c_double * 3),
+                ("nbr", ctypes.c_int32 * 6), ("per", ctypes.c_int32 * 6),
+                ("bc", ctypes.c_int32 * 6)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ip, ctypes.POINTER(OrcBlock), ip, ip, ip, ip]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_density.argtypes = [vp, dp]
+        L.orc_coeff.argtypes = [vp]
+        L.orc_vcycle.argtypes = [vp, dp, dp]
+        L.orc_solve.restype = ip
+        L.orc_solve.argtypes = [vp, dp, dp, ctypes.c_double, ip, ip, dp, dp]
+        L.orc_dims.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_int32)]
+        for name in ("orc_phi", "orc_rhs", "orc_dens"):
+            getattr(L, name).restype = dp
+            getattr(L, name).argtypes = [vp, ip, ip]
+        L.orc_cof.restype = dp
+        L.orc_cof.argtypes = [vp, ip, ip, ip]
+        for name in ("orc_relax",):
+            getattr(L, name).argtypes = [vp, ip, ip]
+        for name in ("orc_restrict", "orc_correct", "orc_exchange", "orc_bound"):
+            getattr(L, name).argtypes = [vp, ip]
+        L.orc_residual.argtypes = [vp, ip, dp]
+        _lib = L
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _concat(parts) -> np.ndarray:
+    return np.ascontiguousarray(np.concatenate([np.ravel(p).astype(np.float64) for p in parts]))
+
+
+class OracleMG:
+    """FP64 multigrid of the reference, restated on the CPU."""
+
+    def __init__(self, blocks, levels: int, lsm: bool = False, nu_pre: int = 2, nu_post: int = 2):
+        self.blocks = list(blocks)
+        self.levels = levels
+        self.lsm = lsm
+        arr = (OrcBlock * len(self.blocks))()
+        for i, b in enumerate(self.blocks):
+            arr[i].n[:] = list(b.n)
+            arr[i].h[:] = list(b.h)
+            arr[i].nbr[:] = list(b.nbr)
+            arr[i].per[:] = list(b.per)
+            arr[i].bc[:] = list(b.bc)
+        self._h = lib().orc_create(len(self.blocks), arr, levels, int(lsm), nu_pre, nu_post)
+        if not self._h:
+            raise ValueError("oracle rejected the block set (extent not divisible by 2**(L-1)?)")
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def dims(self, block: int, level: int):
+        out = (ctypes.c_int32 * 3)()
+        lib().orc_dims(self._h, block, level, out)
+        return tuple(out)
+
+    def set_density(self, dens_blocks):
+        d = _concat(dens_blocks)
+        lib().orc_set_density(self._h, _dptr(d))
+
+    def coeff(self):
+        lib().orc_coeff(self._h)
+
+    def vcycle(self, rhs_blocks, pp_blocks=None):
+        """One mgkcyc; returns pp per block with its ghost layer, shaped (z+2, y+2, x+2)."""
+        rhs = _concat(rhs_blocks)
+        shapes = [tuple(x + 2 for x in reversed(self.dims(b, 0))) for b in range(len(self.blocks))]
+        if pp_blocks is None:
+            pp = np.zeros(sum(int(np.prod(s)) for s in shapes))
+        else:
+            pp = _concat(pp_blocks)
+        lib().orc_vcycle(self._h, _dptr(rhs), _dptr(pp))
+        out, o = [], 0
+        for s in shapes:
+            m = int(np.prod(s))
+            out.append(pp[o:o + m].reshape(s).copy())
+            o += m
+        return out
+
+    def solve(self, f_blocks, tol: float, maxcy: int, min_cycles: int = 1):
+        """Poisson-form outer loop from p = 0.  Returns (p_blocks, rho, rmax, cycles)."""
+        f = _concat(f_blocks)
+        p = np.zeros_like(f)
+        rho = np.zeros(maxcy + 1)
+        rmax = np.zeros(maxcy + 1)
+        k = lib().orc_solve(self._h, _dptr(f), _dptr(p), tol, maxcy, min_cycles,
+                            _dptr(rho), _dptr(rmax))
+        return self._split(p), rho[:k + 1].copy(), rmax[:k + 1].copy(), k
+
+    def _split(self, flat):
+        out, o = [], 0
+        for b in range(len(self.blocks)):
+            nx, ny, nz = self.dims(b, 0)
+            m = nx * ny * nz
+            out.append(flat[o:o + m].reshape(nz, ny, nx).copy())
+            o += m
+        return out
+
+    # -- unit access (views into oracle memory; copy before the next call) --
+    def phi(self, block: int, level: int) -> np.ndarray:
+        nx, ny, nz = self.dims(block, level)
+        p = lib().orc_phi(self._h, block, level)
+        return np.ctypeslib.as_array(p, shape=((nz + 2) * (ny + 2) * (nx + 2),)).reshape(nz + 2, ny + 2, nx + 2)
+
+    def rhs(self, block: int, level: int) -> np.ndarray:
+        nx, ny, nz = self.dims(block, level)
+        p = lib().orc_rhs(self._h, block, level)
+        return np.ctypeslib.as_array(p, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
+
+    def cof(self, block: int, level: int, which: int) -> np.ndarray:
+        nx, ny, nz = self.dims(block, level)
+        p = lib().orc_cof(self._h, block, level, which)
+        return np.ctypeslib.as_array(p, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
+
+    def dens(self, block: int, level: int) -> np.ndarray:
+        nx, ny, nz = self.dims(block, level)
+        p = lib().orc_dens(self._h, block, level)
+        return np.ctypeslib.as_array(p, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
+
+    def relax(self, level: int, sweeps: int):
+        lib().orc_relax(self._h, level, sweeps)
+
+    def restrict(self, coarse_level: int):
+        lib().orc_restrict(self._h, coarse_level)
+
+    def correct(self, fine_level: int):
+        lib().orc_correct(self._h, fine_level)
+
+    def exchange(self, level: int):
+        lib().orc_exchange(self._h, level)
+
+    def bound(self, level: int):
+        lib().orc_bound(self._h, level)
+
+    def residual(self, level: int):
+        tot = sum(int(np.prod(self.dims(b, level))) for b in range(len(self.blocks)))
+        r = np.zeros(tot)
+        lib().orc_residual(self._h, level, _dptr(r))
+        out, o = [], 0
+        for b in range(len(self.blocks)):
+            nx, ny, nz = self.dims(b, level)
+            m = nx * ny * nz
+            out.append(r[o:o + m].reshape(nz, ny, nx).copy())
+            o += m
+        return out
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int = 1,
+            lsm: bool = False, dens_blocks=None, nu_pre: int = 2, nu_post: int = 2):
+    """Run the compiled reference driver.  Returns (p_blocks, rho, rmax, cycles, pp1_blocks)."""
+    if not have_ref():
+        raise RuntimeError("compiled reference not built (make -C oracle ref)")
+    with tempfile.TemporaryDirectory() as td:
+        frhs = os.path.join(td, "rhs.bin")
+        _concat(f_blocks).tofile(frhs)
+        fden = "-"
+        if lsm:
+            fden = os.path.join(td, "dens.bin")
+            _concat(dens_blocks).tofile(fden)
+        fout = os.path.join(td, "out.bin")
+        ctl = os.path.join(td, "ctl.txt")
+        with open(ctl, "w") as fh:
+            nbx, nby, nbz = grid.shape
+            fh.write(f"{nbx} {nby} {nbz} {levels} {int(lsm)} {maxcy} {min_cycles} {nu_pre} {nu_post}\n")
+            fh.write(f"{tol!r}\n")
+            fh.write(" ".join(str(c) for c in grid.bc) + "\n")
+            fh.write(" ".join(map(str, grid.ex)) + "\n")
+            fh.write(" ".join(map(str, grid.ey)) + "\n")
+            fh.write(" ".join(map(str, grid.ez)) + "\n")
+            fh.write(" ".join(repr(float(x)) for x in grid.h) + "\n")
+            fh.write(frhs + "\n" + fden + "\n" + fout + "\n")
+        subprocess.run([REF_BIN, ctl], check=True, capture_output=True, timeout=600)
+        raw = np.fromfile(fout, dtype=np.float64)
+    k = int(raw[0])
+    o = 1
+    rho = raw[o:o + maxcy + 1]; o += maxcy + 1
+    rmax = raw[o:o + maxcy + 1]; o += maxcy + 1
+    p, pp1 = [], []
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        m = nx * ny * nz
+        p.append(raw[o:o + m].reshape(nz, ny, nx).copy()); o += m
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        m = (nx + 2) * (ny + 2) * (nz + 2)
+        pp1.append(raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy()); o += m
+    return p, rho[:k + 1].copy(), rmax[:k + 1].copy(), k, pp1
