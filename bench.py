@@ -1,0 +1,182 @@
+"""Benchmark: pressure-Poisson multigrid solves/s to 1e-8 relative residual on
+one MI355X, FP64 (bitwise the reference's V-cycle) and mixed precision.
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--levels 7] [--no-cpu-baseline]
+
+A step is one complete solve -- from p = 0 to ||f - A p|| / ||f|| <= 1e-8 --
+of the 256^3 single-block channel problem (periodic x, z; Neumann walls in
+y; RHS = divergence of seeded face velocities, mean removed), with the RHS
+already resident in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "MG solves/s to 1e-8 resid (mixed vs FP64), HBM GB/s % peak, p50/p99 ms"
+
+
+def algorithmic_bytes_per_cycle(n_cells_levels, w, c=0):
+    """Streaming model per V(2,2) cycle: sweeps 4(3+c) w N_l, restriction and
+    prolongation (4.25+c) w N_l below the coarsest, FP64 outer (32+2w+8c) N_1."""
+    nu = 4
+    L = len(n_cells_levels)
+    b_inner = sum(nu * (3 + c) * n * w for n in n_cells_levels)
+    b_inner += sum((4.25 + c) * n_cells_levels[l] * w for l in range(L - 1))
+    b_outer = (32 + 2 * w + 8 * c) * n_cells_levels[0]
+    return b_inner + b_outer
+
+
+def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_kernel):
+    import torch
+
+    from multiflow3d_amd import PressureMultigrid
+    from multiflow3d_amd import _lib
+
+    dev = torch.device("cuda:0")
+    f_dev = torch.from_numpy(np.ascontiguousarray(np.concatenate([a.ravel() for a in f_host]))).to(dev)
+    p_dev = torch.zeros_like(f_dev)
+    mg = PressureMultigrid(grid, levels, maxcy=maxcy, eps=tol, precision=precision)
+    for _ in range(warmup):
+        mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr())
+    torch.cuda.synchronize()
+    if time_kernel:
+        mg.time_kernel(_lib.MF3D_K_SMOOTH)
+    lat = []
+    res = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        res = mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr())
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    launches, kms = mg.kernel_time() if time_kernel else (0, 0.0)
+    mg.time_kernel(0)
+    p = p_dev.cpu().numpy()
+    out = dict(ms=(t1 - t0) * 1e3 / steps, lat=lat, iters=res.iters, rho=float(res.hist[-1]),
+               status=res.status, p=p, launches=launches, kernel_ms=kms)
+    mg.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=7)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--maxcy", type=int, default=60)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent solves shard by request; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from multiflow3d_amd import topology as T
+    from multiflow3d_amd import workloads as W
+
+    n, L = args.n, args.levels
+    grid = T.single_block(n, [1.0 / n] * 3, T.CHANNEL_BC)
+    seed = W.BASE_SEED + 2
+    f_host = W.split_blocks(grid, W.global_rhs(grid, seed))
+    levels_cells = [(n >> l) ** 3 for l in range(L)]
+
+    r64 = run_gpu(grid, L, f_host, "fp64", args.steps, args.warmup, args.tol, args.maxcy, True)
+    print(f"# fp64: {r64['ms']:.3f} ms/solve, {r64['iters']} cycles, rho={r64['rho']:.3e}", file=sys.stderr, flush=True)
+    rmx = run_gpu(grid, L, f_host, "mixed", args.steps, args.warmup, args.tol, args.maxcy, True)
+    print(f"# mixed: {rmx['ms']:.3f} ms/solve, {rmx['iters']} cycles, rho={rmx['rho']:.3e}", file=sys.stderr, flush=True)
+
+    def rel_l2_mean_removed(a, b):
+        a = a - a.mean()
+        b = b - b.mean()
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    mixed_err = rel_l2_mean_removed(rmx["p"], r64["p"])
+
+    # dominant kernel: one colour of the red-black sweep on the finest level
+    # algorithmic bytes per launch = N/2 reads of the other colour, N/2 RHS
+    # reads and N/2 writes of this colour, w bytes each
+    N0 = levels_cells[0]
+
+    def roofline(r, w):
+        if not r["launches"]:
+            return None
+        avg_s = r["kernel_ms"] / r["launches"] * 1e-3
+        achieved = 1.5 * w * N0 / avg_s / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_rb (one colour, finest level)", "avg_us": round(avg_s * 1e6, 2),
+                "bytes_per_launch": int(1.5 * w * N0)}
+
+    b64 = algorithmic_bytes_per_cycle(levels_cells, 8) * r64["iters"]
+    bmx = algorithmic_bytes_per_cycle(levels_cells, 4) * rmx["iters"]
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.oracle import OracleMG
+        o = OracleMG(grid.blocks, L)
+        o.coeff()
+        t0 = time.perf_counter()
+        p_o, rho_o, _, k_o = o.solve(f_host, args.tol, args.maxcy)
+        t_cpu = time.perf_counter() - t0
+        o.close()
+        fp64_bitwise = bool(np.array_equal(p_o[0].ravel(), r64["p"]))
+        cpu = {"value": round(1.0 / t_cpu, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"one full FP64 solve of the same {n}^3 problem ({k_o} cycles, "
+                         f"{t_cpu:.1f} s) by the C restatement of the reference (oracle/mg_oracle.c), "
+                         f"single thread",
+               "fp64_gpu_bitwise_equal": fp64_bitwise}
+
+    lat = np.array(rmx["lat"])
+    line = {
+        "metric": METRIC,
+        "value": round(1e3 / rmx["ms"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rmx["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner cycle, f64 outer residual/update",
+        "data": "synthetic (SplitMix64 face velocities -> divergence, mean removed)",
+        "config": {"workload": f"single {n}^3 block, periodic x/z, Neumann y, V(2,2) RBGS, "
+                               f"{L} levels, to {args.tol:g} relative residual",
+                   "cells": N0, "levels": L, "tol": args.tol},
+        "roofline": roofline(rmx, 4),
+        "cpu_baseline": cpu,
+        "fp64": {"solves_per_s": round(1e3 / r64["ms"], 4), "ms_per_solve": round(r64["ms"], 4),
+                 "cycles": r64["iters"], "final_rho": r64["rho"],
+                 "alg_GBps": round(b64 / (r64["ms"] * 1e-3) / 1e9, 1),
+                 "alg_frac_of_peak": round(b64 / (r64["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                 "roofline": roofline(r64, 8)},
+        "mixed": {"solves_per_s": round(1e3 / rmx["ms"], 4), "ms_per_solve": round(rmx["ms"], 4),
+                  "cycles": rmx["iters"], "final_rho": rmx["rho"],
+                  "alg_GBps": round(bmx / (rmx["ms"] * 1e-3) / 1e9, 1),
+                  "alg_frac_of_peak": round(bmx / (rmx["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                  "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                  "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                  "rel_l2_vs_fp64": mixed_err},
+        "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

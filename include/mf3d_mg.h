@@ -1,0 +1,133 @@
+/*
+ * mf3d_mg.h -- C ABI of the MI355X (gfx950) pressure-Poisson multigrid that
+ * drops in for MultiFlow3D's FP64 solver path:
+ *
+ *   subroutine coeff       mgsolver.for:2-222       -> mf3d_mg_create / mf3d_mg_set_density
+ *   subroutine mgkcyc      mgsolver.for:224-369     -> mf3d_mg_vcycle
+ *   subroutine newsolv_mg  newsolv_mg.for:2-102     -> mf3d_mg_solve (Poisson form)
+ *
+ * The reference routines take no arguments: their inputs are module state
+ * (module_vars.for:11-15, module_multidata.for:15-120).  Here the same state
+ * is passed explicitly: one mf3d_block_desc per dom(ib), and the MG scalars
+ * (ngrd_gl, mg_itrsch, irestr, iproln, maxcy, eps, L_LSM) in mf3d_mg_cfg.
+ *
+ * Arrays are plain pointers to FP64 data in the caller's layout:
+ * column-major (i fastest), dims ld[0..2], interior starting at index
+ * `ghost` on every axis -- the reference's dom(ib)%p / %pp / %su / %sup /
+ * %dens with ld = ttc_* and ghost = pl (initial.for:199-204).
+ *
+ * Return codes: 0 converged / success, 1 not converged within maxcy
+ * (newsolv_mg.for:60-61 prints and continues), 2 diverged (rmax > 100,
+ * newsolv_mg.for:64-83 stops), < 0 an argument, setup or device error
+ * (text via mf3d_mg_last_error).
+ *
+ * A handle owns its device memory and one HIP stream; it is not thread-safe.
+ */
+#ifndef MF3D_MG_H
+#define MF3D_MG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF3D_MG_ABI_VERSION 1
+
+enum {
+    MF3D_OK = 0,
+    MF3D_NOT_CONVERGED = 1,
+    MF3D_DIVERGED = 2,
+    MF3D_EINVAL = -22,
+    MF3D_ENOTSUP = -95,
+    MF3D_ENOMEM = -12,
+    MF3D_EDEVICE = -5
+};
+
+enum { MF3D_FP64 = 0, MF3D_MIXED = 1 };
+
+typedef struct mf3d_mg mf3d_mg;
+
+/* One block: the fields of dom(ib) the multigrid reads
+ * (module_multidata.for:16-28,36,116; alloc_dom.for:456-616,972-1150).
+ * Faces are ordered W,E,S,N,B,T.  Edge and corner neighbours (the
+ * corprev, cornext, edgprev and edgnext fields) are not passed: on a
+ * conforming block grid they follow from the face neighbours. */
+typedef struct {
+    int32_t n[3];    /* interior cells along i, j, k (iep-isp+1, ...) */
+    int32_t ghost;   /* ghost layers in the caller's arrays (pl) */
+    int32_t ld[3];   /* caller array dims (ttc_i, ttc_j, ttc_k); 0 -> n + 2*ghost */
+    double  h[3];    /* dx, dy, dz */
+    int32_t nbr[6];  /* iprev, inext, jprev, jnext, kprev, knext: block index or -1 */
+    int32_t per[6];  /* per_ip, per_in, per_jp, per_jn, per_kp, per_kn: block index or -1 */
+    int32_t bc[6];   /* bc_west .. bc_top; 5 = periodic, any other code = Neumann for p */
+} mf3d_block_desc;
+
+typedef struct {
+    int32_t nblocks;       /* nbp */
+    int32_t levels;        /* ngrd_gl; every extent must divide by 2^(levels-1) */
+    int32_t smoother;      /* mg_itrsch; only 3 (red-black Gauss-Seidel) is supported */
+    int32_t nu_pre;        /* irestr */
+    int32_t nu_post;       /* iproln */
+    int32_t maxcy;         /* maximum outer cycles */
+    double  eps;           /* stop when ||f - A p||_2 / ||f||_2 <= eps */
+    int32_t min_cycles;    /* newsolv_mg requires iter > 1, i.e. 2 */
+    int32_t precision;     /* MF3D_FP64 (bitwise the reference) or MF3D_MIXED */
+    int32_t variable_coef; /* 0: 1/h^2 coefficients; 1: LSM density coefficients */
+    int32_t device;        /* HIP device ordinal */
+} mf3d_mg_cfg;
+
+/* coeff sizing + constant coefficients (mgsolver.for:2-222, localparameters.for:200-237). */
+int mf3d_mg_create(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, mf3d_mg **out);
+
+/* coeff, L_LSM branch (mgsolver.for:69-181): cell density dom(ib)%dens of
+ * every block, caller layout.  Builds the whole level hierarchy on device. */
+int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks);
+
+/* mgkcyc (mgsolver.for:224-369): one V-cycle on rhs (su, or sup under LSM);
+ * pp is read and written including one ghost layer around the interior,
+ * so the caller arrays need ghost >= 1. */
+int mf3d_mg_vcycle(mf3d_mg *mg, const double *const *rhs_blocks, double *const *pp_blocks);
+
+/* newsolv_mg in Poisson form: p = 0, then repeat { one V-cycle on
+ * r = f - A p; p += e } until ||r||/||f|| <= eps (and at least min_cycles
+ * cycles) or maxcy cycles.  rmax_out gets the reference's own absolute
+ * criterion max|r|*dx*dy*dz (max|r| under LSM, whose RHS carries the cell
+ * volume); hist (maxcy+1 doubles, may be NULL) gets ||r_k||/||f|| for
+ * k = 0..iters.  p is written over the interior only. */
+int mf3d_mg_solve(mf3d_mg *mg, const double *const *f_blocks, double *const *p_blocks,
+                  double *rmax_out, int32_t *iters_out, double *hist);
+
+/* Device-resident solve: f_dev and p_dev are device pointers to all blocks'
+ * interiors concatenated in block order (i fastest within a block).  Runs
+ * on `stream` (a hipStream_t; NULL = the handle's own stream) and returns
+ * when the solve is complete.  No host transfer of fields. */
+int mf3d_mg_solve_device(mf3d_mg *mg, const double *f_dev, double *p_dev, void *stream,
+                         double *rmax_out, int32_t *iters_out, double *hist);
+
+/* Interior cells of all blocks (length of the device-resident vectors). */
+int64_t mf3d_mg_cells(const mf3d_mg *mg);
+
+/* Device bytes held by the handle (for HBM-budgeted admission). */
+int64_t mf3d_mg_device_bytes(const mf3d_mg *mg);
+
+/* Bytes a handle with this configuration would hold, without creating it. */
+int64_t mf3d_mg_estimate_bytes(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks);
+
+/* Measurement hook (no counterpart in the reference; for benchmarks):
+ * while enabled, every finest-level launch of the chosen kernel is
+ * bracketed by HIP events on the stream it runs on.  mf3d_mg_time_kernel
+ * selects the kernel (MF3D_K_*; 0 = off) and resets the counters;
+ * mf3d_mg_kernel_time returns the launches seen and their summed time. */
+enum { MF3D_K_OFF = 0, MF3D_K_SMOOTH = 1, MF3D_K_OUTER = 2, MF3D_K_RESTRICT = 3, MF3D_K_PROLONG = 4 };
+int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel);
+int mf3d_mg_kernel_time(mf3d_mg *mg, int64_t *launches, double *total_ms);
+
+const char *mf3d_mg_last_error(const mf3d_mg *mg);
+void mf3d_mg_destroy(mf3d_mg *mg);
+int mf3d_mg_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

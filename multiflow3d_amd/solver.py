@@ -1,0 +1,243 @@
+"""Host-side mirror of the reference's pressure-solver interface, on top of
+the C ABI.
+
+The reference exposes three argument-less subroutines over module state:
+
+    coeff        mgsolver.for:2-222      coefficient hierarchy
+    mgkcyc       mgsolver.for:224-369    one V-cycle on su (sup) into pp
+    newsolv_mg   newsolv_mg.for:2-102    outer defect-correction loop
+
+``PressureMultigrid`` keeps those names; the module state becomes the
+constructor's arguments (a block grid and the MG scalars of control.cin)
+and explicit arrays.  Error behaviour follows newsolv_mg: not converging
+within maxcy warns and returns (newsolv_mg.for:60-61), rmax > 100 raises
+(the reference stops, newsolv_mg.for:64-83).
+
+Block arrays are numpy float64 arrays shaped (nz, ny, nx) -- the memory
+order of the reference's column-major p(i,j,k) -- either interior-only or
+with a one-cell ghost layer, (nz+2, ny+2, nx+2).
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .topology import Block, BlockGrid
+
+
+class MGError(RuntimeError):
+    pass
+
+
+class Diverged(MGError):
+    pass
+
+
+class NotConverged(UserWarning):
+    pass
+
+
+@dataclass
+class SolveResult:
+    status: int          # 0 converged, 1 not converged
+    rmax: float          # max |r| dV after the last cycle (reference criterion)
+    iters: int
+    hist: np.ndarray     # ||r_k|| / ||f||, k = 0..iters
+
+
+PRECISIONS = {"fp64": _lib.MF3D_FP64, "mixed": _lib.MF3D_MIXED}
+
+
+def _descs(blocks: Sequence[Block], ghost: int):
+    arr = (_lib.BlockDesc * len(blocks))()
+    for i, b in enumerate(blocks):
+        arr[i].n[:] = list(b.n)
+        arr[i].ghost = ghost
+        arr[i].ld[:] = [b.n[0] + 2 * ghost, b.n[1] + 2 * ghost, b.n[2] + 2 * ghost]
+        arr[i].h[:] = list(b.h)
+        arr[i].nbr[:] = list(b.nbr)
+        arr[i].per[:] = list(b.per)
+        arr[i].bc[:] = list(b.bc)
+    return arr
+
+
+def make_cfg(nblocks: int, levels: int, *, nu_pre=2, nu_post=2, maxcy=30, eps=1e-8,
+             min_cycles=1, precision="fp64", variable_coef=False, device=0, smoother=3):
+    c = _lib.Cfg()
+    c.nblocks, c.levels, c.smoother = nblocks, levels, smoother
+    c.nu_pre, c.nu_post, c.maxcy = nu_pre, nu_post, maxcy
+    c.eps, c.min_cycles = float(eps), min_cycles
+    c.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    c.variable_coef = int(bool(variable_coef))
+    c.device = device
+    return c
+
+
+def estimate_bytes(grid_or_blocks, levels: int, **kw) -> int:
+    blocks = grid_or_blocks.blocks if isinstance(grid_or_blocks, BlockGrid) else list(grid_or_blocks)
+    cfg = make_cfg(len(blocks), levels, **kw)
+    v = _lib.lib().mf3d_mg_estimate_bytes(ctypes.byref(cfg), _descs(blocks, 1))
+    if v < 0:
+        raise MGError("invalid configuration")
+    return int(v)
+
+
+class PressureMultigrid:
+    """One solver handle: device-resident level hierarchy for a block set."""
+
+    GHOST = 1   # caller arrays carry one ghost layer, as the reference's pp (pl = 1)
+
+    def __init__(self, grid_or_blocks, levels: int, *, nu_pre: int = 2, nu_post: int = 2,
+                 maxcy: int = 30, eps: float = 1e-8, min_cycles: int = 1,
+                 precision: str = "fp64", variable_coef: bool = False, device: int = 0,
+                 smoother: int = 3):
+        self.blocks: List[Block] = (grid_or_blocks.blocks if isinstance(grid_or_blocks, BlockGrid)
+                                    else list(grid_or_blocks))
+        self.levels = levels
+        self.maxcy = maxcy
+        self.variable_coef = variable_coef
+        self.cfg = make_cfg(len(self.blocks), levels, nu_pre=nu_pre, nu_post=nu_post,
+                            maxcy=maxcy, eps=eps, min_cycles=min_cycles, precision=precision,
+                            variable_coef=variable_coef, device=device, smoother=smoother)
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        rc = L.mf3d_mg_create(ctypes.byref(self.cfg), _descs(self.blocks, self.GHOST), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = L.mf3d_mg_last_error(h).decode() if h.value else ""
+            self.close()
+            raise MGError(f"mf3d_mg_create failed ({rc}): {msg}")
+
+    # ------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().mf3d_mg_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _err(self) -> str:
+        return _lib.lib().mf3d_mg_last_error(self._h).decode()
+
+    @property
+    def cells(self) -> int:
+        return int(_lib.lib().mf3d_mg_cells(self._h))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.lib().mf3d_mg_device_bytes(self._h))
+
+    # ------------------------------------------------------------------
+    def _ghosted(self, arrays, copy_in=True):
+        out = []
+        for b, a in zip(self.blocks, arrays):
+            nx, ny, nz = b.n
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape == (nz + 2, ny + 2, nx + 2):
+                out.append(np.ascontiguousarray(a))
+            elif a.shape == (nz, ny, nx):
+                g = np.zeros((nz + 2, ny + 2, nx + 2))
+                if copy_in:
+                    g[1:-1, 1:-1, 1:-1] = a
+                out.append(g)
+            else:
+                raise ValueError(f"block array shape {a.shape} does not match extent {b.n}")
+        return out
+
+    @staticmethod
+    def _ptrs(arrays):
+        return (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+    def coeff(self, density=None):
+        """Coefficient hierarchy; under LSM from the cell density of each block."""
+        if density is None:
+            if self.variable_coef:
+                raise MGError("variable-coefficient handle needs a density")
+            return
+        dens = self._ghosted(density)
+        rc = _lib.lib().mf3d_mg_set_density(self._h, self._ptrs(dens))
+        if rc != 0:
+            raise MGError(f"coeff failed ({rc}): {self._err()}")
+
+    def mgkcyc(self, rhs, pp):
+        """One V-cycle on ``rhs``; ``pp`` (ghosted arrays) is updated in place and returned."""
+        r = self._ghosted(rhs)
+        for b, a in zip(self.blocks, pp):
+            if a.shape != (b.n[2] + 2, b.n[1] + 2, b.n[0] + 2) or a.dtype != np.float64 \
+                    or not a.flags.c_contiguous:
+                raise ValueError("pp blocks must be C-contiguous float64 with one ghost layer")
+        rc = _lib.lib().mf3d_mg_vcycle(self._h, self._ptrs(r), self._ptrs(list(pp)))
+        if rc != 0:
+            raise MGError(f"mgkcyc failed ({rc}): {self._err()}")
+        return pp
+
+    def _finish(self, rc, rmax, iters):
+        hist = self._hist[: iters.value + 1].copy()
+        res = SolveResult(status=rc, rmax=rmax.value, iters=iters.value, hist=hist)
+        if rc == _lib.MF3D_DIVERGED:
+            raise Diverged(f"BIG RMAX!! STOP!!!!! rmax={rmax.value!r}")
+        if rc < 0:
+            raise MGError(f"solve failed ({rc}): {self._err()}")
+        if rc == _lib.MF3D_NOT_CONVERGED:
+            warnings.warn(f"not converged!! {self.maxcy} {rmax.value!r}", NotConverged)
+        return res
+
+    def newsolv_mg(self, f, p: Optional[list] = None):
+        """Solve A p = f from p = 0; returns (p blocks interior, SolveResult)."""
+        fg = self._ghosted(f)
+        pg = [np.zeros_like(a) for a in fg]
+        rmax = ctypes.c_double(0.0)
+        iters = ctypes.c_int32(0)
+        self._hist = np.zeros(self.maxcy + 1)
+        rc = _lib.lib().mf3d_mg_solve(self._h, self._ptrs(fg), self._ptrs(pg), ctypes.byref(rmax),
+                                      ctypes.byref(iters),
+                                      self._hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        res = self._finish(rc, rmax, iters)
+        out = [a[1:-1, 1:-1, 1:-1].copy() for a in pg]
+        if p is not None:
+            for dst, src in zip(p, out):
+                if dst.shape == src.shape:
+                    dst[...] = src
+                else:
+                    dst[1:-1, 1:-1, 1:-1] = src
+        return out, res
+
+    def time_kernel(self, kernel: int):
+        """Enable the finest-level timing hook for one kernel (MF3D_K_*; 0 = off)."""
+        rc = _lib.lib().mf3d_mg_time_kernel(self._h, int(kernel))
+        if rc != 0:
+            raise MGError(f"time_kernel failed ({rc})")
+
+    def kernel_time(self):
+        """(launches, total_ms) recorded by the timing hook."""
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0.0)
+        _lib.lib().mf3d_mg_kernel_time(self._h, ctypes.byref(n), ctypes.byref(ms))
+        return int(n.value), float(ms.value)
+
+    def solve_device(self, f_dev_ptr: int, p_dev_ptr: int, stream_ptr: int = 0) -> SolveResult:
+        """Device-resident solve on interior-packed device vectors (e.g. torch
+        tensors' data_ptr()); the fields never leave HBM."""
+        rmax = ctypes.c_double(0.0)
+        iters = ctypes.c_int32(0)
+        self._hist = np.zeros(self.maxcy + 1)
+        rc = _lib.lib().mf3d_mg_solve_device(
+            self._h, ctypes.c_void_p(f_dev_ptr), ctypes.c_void_p(p_dev_ptr),
+            ctypes.c_void_p(stream_ptr), ctypes.byref(rmax), ctypes.byref(iters),
+            self._hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return self._finish(rc, rmax, iters)

@@ -1,0 +1,99 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle.
+
+FP64 results must be bitwise identical to the oracle (same operations in
+the same order, no FMA contraction); residual histories may differ only by
+the summation order of the norms (relative 1e-12 here, the bar is 1e-10).
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+from multiflow3d_amd import PressureMultigrid, NotConverged
+from oracle.oracle import OracleMG
+
+from .cases import SMALL, case, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(grid, L, lsm, dens):
+    o = OracleMG(grid.blocks, L, lsm=lsm)
+    if lsm:
+        o.set_density(dens)
+    o.coeff()
+    return o
+
+
+def _gpu(grid, L, lsm, dens, **kw):
+    mg = PressureMultigrid(grid, L, variable_coef=lsm, **kw)
+    if lsm:
+        mg.coeff(dens)
+    return mg
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_fixed_cycles_bitwise(name):
+    grid, L, lsm = case(name)
+    f, dens = inputs(grid, lsm)
+    ncyc = 5
+    p_o, rho_o, rmax_o, k_o = _oracle(grid, L, lsm, dens).solve(f, 0.0, ncyc)
+    with _gpu(grid, L, lsm, dens, maxcy=ncyc, eps=0.0) as mg:
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", NotConverged)
+            p_g, res = mg.newsolv_mg(f)
+    assert res.iters == k_o == ncyc
+    np.testing.assert_allclose(res.hist, rho_o, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(res.rmax, rmax_o[-1], rtol=1e-12, atol=0)
+    for a, b in zip(p_g, p_o):
+        assert np.array_equal(a, b), f"{name}: p differs, max {np.max(np.abs(a - b))}"
+
+
+@pytest.mark.parametrize("name", ["single16_L4", "multi_uneven_L3", "lsm_multi_L3",
+                                  "periodic_multi_L3", "mixed_bc_multi_L3"])
+def test_vcycle_bitwise_with_ghosts(name):
+    """mgkcyc alone: pp after one V-cycle, ghost layer included."""
+    grid, L, lsm = case(name)
+    f, dens = inputs(grid, lsm, seed=7)
+    pp_o = _oracle(grid, L, lsm, dens).vcycle(f)
+    with _gpu(grid, L, lsm, dens) as mg:
+        pp = [np.zeros((b.n[2] + 2, b.n[1] + 2, b.n[0] + 2)) for b in grid.blocks]
+        mg.mgkcyc(f, pp)
+    for a, b in zip(pp, pp_o):
+        assert np.array_equal(a, b), f"{name}: pp differs, max {np.max(np.abs(a - b))}"
+
+
+@pytest.mark.parametrize("name", ["single16_L4", "multi_uneven_L3", "lsm16_L4"])
+def test_converged_solve_matches(name):
+    grid, L, lsm = case(name)
+    f, dens = inputs(grid, lsm)
+    p_o, rho_o, _, k_o = _oracle(grid, L, lsm, dens).solve(f, 1e-8, 200)
+    with _gpu(grid, L, lsm, dens, maxcy=200, eps=1e-8) as mg:
+        p_g, res = mg.newsolv_mg(f)
+    assert res.status == 0 and res.iters == k_o
+    assert res.hist[-1] <= 1e-8
+    for a, b in zip(p_g, p_o):
+        assert np.array_equal(a, b)
+
+
+def test_min_cycles_and_status():
+    grid, L, lsm = case("single16_L4")
+    f, _ = inputs(grid, False)
+    with _gpu(grid, L, False, None, maxcy=3, eps=1e-30) as mg:
+        with pytest.warns(NotConverged):
+            _, res = mg.newsolv_mg(f)
+        assert res.status == 1 and res.iters == 3
+    with _gpu(grid, L, False, None, maxcy=10, eps=1.0, min_cycles=2) as mg:
+        _, res = mg.newsolv_mg(f)
+        assert res.status == 0 and res.iters == 2   # newsolv_mg.for:56: iter > 1
+
+
+def test_deterministic_repeat():
+    grid, L, lsm = case("multi_uneven_L3")
+    f, _ = inputs(grid, False)
+    with _gpu(grid, L, False, None, maxcy=20, eps=1e-9) as mg:
+        p1, r1 = mg.newsolv_mg(f)
+        p2, r2 = mg.newsolv_mg(f)
+    assert np.array_equal(r1.hist, r2.hist)
+    for a, b in zip(p1, p2):
+        assert np.array_equal(a, b)
