@@ -9,3 +9,13 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X; runs the HIP library")
     config.addinivalue_line("markers", "slow: full-size cases")
+
+
+def _ensure_oracle():
+    import subprocess
+    lib = os.path.join(ROOT, "oracle", "liborc_mg.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+_ensure_oracle()
