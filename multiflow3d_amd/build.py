@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmf3d_mg.so")
 SRC = [os.path.join(HERE, "csrc", "mf3d_mg.hip")]
-DEPS = SRC + [os.path.join(HERE, "csrc", "mg_kernels.hpp"), os.path.join(ROOT, "include", "mf3d_mg.h")]
+DEPS = SRC + [os.path.join(HERE, "csrc", h) for h in ("mg_common.hpp", "mg_outer.hpp", "mg_setup.hpp", "mg_sweep.hpp", "mg_tail.hpp", "mg_transfer.hpp")] + [os.path.join(ROOT, "include", "mf3d_mg.h")]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MF3D_OFFLOAD_ARCH", "gfx950")
@@ -44,6 +44,31 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_HOST = os.path.join(FORTRAN_DIR, "_build", "mf3d_host")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING module and the small Fortran host that drives the
+    library through it (fortran/_build/mf3d_host, linked to libmf3d_mg.so)."""
+    out = os.path.join(FORTRAN_DIR, "_build")
+    os.makedirs(out, exist_ok=True)
+    steps = [
+        [FLANG, "-c", "-O2", "-J", out, os.path.join(FORTRAN_DIR, "mf3d_mg_f.f90"),
+         "-o", os.path.join(out, "mf3d_mg_f.o")],
+        [FLANG, "-c", "-O2", "-J", out, "-I", out, os.path.join(FORTRAN_DIR, "mf3d_host.f90"),
+         "-o", os.path.join(out, "mf3d_host.o")],
+        [FLANG, "-o", FORTRAN_HOST, os.path.join(out, "mf3d_host.o"), os.path.join(out, "mf3d_mg_f.o"),
+         f"-L{HERE}", "-lmf3d_mg", "-Wl,-rpath,$ORIGIN/../.."],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return FORTRAN_HOST
+
+
 def build_oracle(with_ref: bool = True) -> None:
     """Build the CPU oracle and, where the reference sources are present, the
     compiled reference driver (test infrastructure, never shipped)."""
@@ -55,3 +80,4 @@ def build_oracle(with_ref: bool = True) -> None:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

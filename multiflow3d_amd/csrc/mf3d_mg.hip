@@ -6,39 +6,54 @@
 // V-cycle (mgkcyc, mgsolver.for:224-369 with kcycle = 1):
 //   level 0: nu_pre sweeps; then down to the coarsest: restrict, nu_pre
 //   sweeps (nu_pre + nu_post on the coarsest, which is only smoothed);
-//   then up: prolong + Neumann copy, nu_post sweeps.
-// One sweep = red colour, black colour, then the ghost gather (mgbound +
+//   then up: prolong (with the Neumann copy), nu_post sweeps.
+// One sweep = both colours in one kernel, then the ghost gather (mgbound +
 // exchangepp + exchbc_mgpp, mgsolver.for:610-611).  No ghost refresh
 // between the two colours, and none of the periodic or inter-block ghosts
 // after prolongation: the reference refreshes neither.
+//
+// A solve captures one cycle (V-cycle + outer step + norm) in a hipGraph,
+// two variants for the two directions of the pressure ping-pong, and
+// replays it in chunks sized by the previous solve; the device raises a
+// `done` flag at convergence, after which the remaining kernels of the
+// chunk return at once.  The host synchronises once per chunk.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/mf3d_mg.h"
-#include "mg_kernels.hpp"
+#include "mg_common.hpp"
+#include "mg_outer.hpp"
+#include "mg_setup.hpp"
+#include "mg_sweep.hpp"
+#include "mg_tail.hpp"
+#include "mg_transfer.hpp"
 
 using namespace mf3d;
 
 namespace {
 
-constexpr int kTileX = 64, kTileY = 4, kRbTileX = 128;
+constexpr int kTileX = 64, kTileY = 4;
 
 struct Level {
     std::vector<BlockLevel> hb;
     BlockLevel *db = nullptr;
-    Tile *t_rb = nullptr, *t_in = nullptr, *t_gh = nullptr;
-    int n_rb = 0, n_in = 0, n_gh = 0;
-    Region *r_full = nullptr, *r_neu = nullptr;
-    int n_full = 0, n_neu = 0, max_full = 0, max_neu = 0;
+    Tile *t_sw = nullptr, *t_in = nullptr, *t_row = nullptr;
+    int n_sw = 0, n_in = 0, n_row = 0;
+    int sw_ty = 4;  // sweep tile rows
+    Region *r_full = nullptr;
+    int n_full = 0, max_full = 0;
     long long cells_gh = 0;  // arena length (all blocks, ghosted)
-    void *phi = nullptr, *rhs = nullptr;
-    void *cx = nullptr, *cy = nullptr, *cz = nullptr;  // inner precision (variable coef)
+    void *phi[2] = {nullptr, nullptr};
+    int cur = 0;  // which phi holds the current potential (host bookkeeping)
+    void *rhs = nullptr;
+    void *cx = nullptr, *cy = nullptr, *cz = nullptr;      // inner precision (variable coef)
     double *dcx = nullptr, *dcy = nullptr, *dcz = nullptr;  // FP64 (variable coef)
     double *dens = nullptr;
 };
@@ -50,19 +65,31 @@ struct mf3d_mg {
     std::vector<mf3d_block_desc> blk;
     hipStream_t stream = nullptr;
     std::vector<Level> lv;
-    double *p_a = nullptr, *p_b = nullptr, *f_g = nullptr;
+    double *pbuf[2] = {nullptr, nullptr};
+    double *f_g = nullptr;
     double *partials = nullptr;
     int nparts = 0;
     SolveState *st = nullptr;
     double *d_hist = nullptr, *d_rmax = nullptr;
-    double *d_io = nullptr;   // device staging for host transfers (packed)
-    double *h_io = nullptr;   // pinned host staging
+    double *d_io = nullptr;  // device staging for host transfers (packed)
+    double *h_io = nullptr;  // pinned host staging
     long long io_len = 0;
     long long cells = 0;
     long long bytes = 0;
     bool density_set = false;
     std::vector<void *> allocs;
     std::string err;
+    // captured cycles: graph g reads pbuf[g], writes pbuf[1-g]
+    hipGraphExec_t cyc[2] = {nullptr, nullptr};
+    // coarse tail: levels lt..L-1 run in LDS, one workgroup per block
+    int lt = 0;
+    BlockLevel *d_tail = nullptr;
+    size_t tail_lds = 0;
+    // outer step tiles (finest level)
+    Tile *t_out = nullptr;
+    int n_out = 0;
+    bool graphs_ready = false;
+    int guess = 0;  // cycles the previous solve needed
     // kernel timing hook
     int tk = 0;
     std::vector<hipEvent_t> ev;
@@ -81,8 +108,6 @@ namespace {
             return MF3D_EDEVICE;                                                       \
         }                                                                              \
     } while (0)
-
-std::string g_create_err;
 
 int dev_alloc(mf3d_mg *mg, void **p, size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
@@ -108,9 +133,7 @@ int upload(mf3d_mg *mg, X **dst, const std::vector<X> &src) {
 
 inline int dim_at(const mf3d_block_desc &b, int d, int l) { return b.n[d] >> l; }
 
-bool is_neumann(const mf3d_block_desc &b, int f) {
-    return b.nbr[f] < 0 && b.bc[f] != 5;
-}
+bool is_neumann(const mf3d_block_desc &b, int f) { return b.nbr[f] < 0 && b.bc[f] != 5; }
 
 // Validate the block set: extents divisible by 2^(L-1), neighbours in range,
 // symmetric and conforming (equal extents across a shared face).
@@ -124,9 +147,10 @@ std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     for (int i = 0; i < c.nblocks; ++i) {
         for (int d = 0; d < 3; ++d) {
             if (b[i].n[d] <= 0) return "block " + std::to_string(i) + ": empty extent";
-            if (b[i].n[d] % div) return "block " + std::to_string(i) +
-                                       ": the given number not correct for mgsolver (extent not "
-                                       "divisible by 2^(levels-1))";
+            if (b[i].n[d] % div)
+                return "block " + std::to_string(i) +
+                       ": the given number not correct for mgsolver (extent not divisible by "
+                       "2^(levels-1))";
             if (!(b[i].h[d] > 0.0)) return "block " + std::to_string(i) + ": non-positive spacing";
             const int ld = b[i].ld[d] ? b[i].ld[d] : b[i].n[d] + 2 * b[i].ghost;
             if (ld < b[i].n[d] + 2 * b[i].ghost) return "block " + std::to_string(i) + ": ld too small";
@@ -152,15 +176,12 @@ std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     return "";
 }
 
-// Ghost regions of one level.  Full exchange: each ghost maps, axis by axis
-// (z, then y, then x, the order in which the reference's face passes
-// propagate edges and corners), to the interior cell it stands for --
-// across a face neighbour, across a periodic partner, onto the block
-// itself for a lone periodic block, or reflected at a Neumann wall.
-// Neumann-only (mgbound after prolongation): reflect the Neumann axes,
-// keep every other coordinate as it is.
-std::vector<Region> build_regions(const std::vector<mf3d_block_desc> &blk, int l,
-                                  bool neumann_only, int *max_cnt) {
+// Ghost regions of one level: each ghost maps, axis by axis (z, then y,
+// then x, the order in which the reference's face passes propagate edges
+// and corners), to the interior cell it stands for -- across a face
+// neighbour, across a periodic partner, onto the block itself for a lone
+// periodic block, or reflected at a Neumann wall.
+std::vector<Region> build_regions(const std::vector<mf3d_block_desc> &blk, int l, int *max_cnt) {
     std::vector<Region> out;
     *max_cnt = 0;
     for (int b = 0; b < (int)blk.size(); ++b) {
@@ -178,35 +199,21 @@ std::vector<Region> build_regions(const std::vector<mf3d_block_desc> &blk, int l
                         else { r.lo[d] = 1; r.cnt[d] = n[d]; }
                         r.map[d] = -1;
                     }
-                    if (neumann_only) {
-                        bool any = false;
-                        for (int d = 0; d < 3; ++d) {
-                            if (!off[d]) continue;
-                            const int f = 2 * d + (off[d] > 0);
-                            if (is_neumann(blk[b], f)) {
-                                r.map[d] = off[d] < 0 ? 1 : n[d];
-                                any = true;
-                            }
+                    int cur = b;
+                    for (int d = 2; d >= 0; --d) {
+                        if (!off[d]) continue;
+                        const int f = 2 * d + (off[d] > 0);
+                        const mf3d_block_desc &B = blk[cur];
+                        if (B.nbr[f] >= 0 || (B.bc[f] == 5 && B.per[f] >= 0)) {
+                            cur = B.nbr[f] >= 0 ? B.nbr[f] : B.per[f];
+                            r.map[d] = off[d] < 0 ? dim_at(blk[cur], d, l) : 1;
+                        } else if (B.bc[f] == 5) {
+                            r.map[d] = off[d] < 0 ? dim_at(B, d, l) : 1;
+                        } else {
+                            r.map[d] = off[d] < 0 ? 1 : dim_at(B, d, l);
                         }
-                        if (!any) continue;
-                        r.src_b = b;
-                    } else {
-                        int cur = b;
-                        for (int d = 2; d >= 0; --d) {
-                            if (!off[d]) continue;
-                            const int f = 2 * d + (off[d] > 0);
-                            const mf3d_block_desc &B = blk[cur];
-                            if (B.nbr[f] >= 0 || (B.bc[f] == 5 && B.per[f] >= 0)) {
-                                cur = B.nbr[f] >= 0 ? B.nbr[f] : B.per[f];
-                                r.map[d] = off[d] < 0 ? dim_at(blk[cur], d, l) : 1;
-                            } else if (B.bc[f] == 5) {
-                                r.map[d] = off[d] < 0 ? dim_at(B, d, l) : 1;
-                            } else {
-                                r.map[d] = off[d] < 0 ? 1 : dim_at(B, d, l);
-                            }
-                        }
-                        r.src_b = cur;
                     }
+                    r.src_b = cur;
                     *max_cnt = std::max(*max_cnt, r.cnt[0] * r.cnt[1] * r.cnt[2]);
                     out.push_back(r);
                 }
@@ -214,16 +221,47 @@ std::vector<Region> build_regions(const std::vector<mf3d_block_desc> &blk, int l
     return out;
 }
 
-std::vector<Tile> build_tiles(const std::vector<BlockLevel> &hb, int tx, bool ghosted) {
+// Cell tiles: 64 x 4 cells of one plane (interior).
+std::vector<Tile> cell_tiles(const std::vector<BlockLevel> &hb) {
     std::vector<Tile> t;
     for (int b = 0; b < (int)hb.size(); ++b) {
         const BlockLevel &g = hb[b];
-        const int lo = ghosted ? 0 : 1;
-        const int h0 = ghosted ? g.n0 + 1 : g.n0, h1 = ghosted ? g.n1 + 1 : g.n1,
-                  h2 = ghosted ? g.n2 + 1 : g.n2;
-        for (int z = lo; z <= h2; ++z)
-            for (int y = lo; y <= h1; y += kTileY)
-                for (int x = lo; x <= h0; x += tx) t.push_back(Tile{b, x, y, z});
+        for (int z = 1; z <= g.n2; ++z)
+            for (int y = 1; y <= g.n1; y += kTileY)
+                for (int x = 1; x <= g.n0; x += kTileX) t.push_back(Tile{b, x, y, z, z});
+    }
+    return t;
+}
+
+// Row tiles: 4 full rows (ghosted range) of one ghosted plane.
+std::vector<Tile> row_tiles(const std::vector<BlockLevel> &hb) {
+    std::vector<Tile> t;
+    for (int b = 0; b < (int)hb.size(); ++b) {
+        const BlockLevel &g = hb[b];
+        for (int z = 0; z <= g.n2 + 1; ++z)
+            for (int y = 0; y <= g.n1 + 1; y += kTileY) t.push_back(Tile{b, 0, y, z, z});
+    }
+    return t;
+}
+
+// Sweep tiles: 64 x ty columns, a chunk of tz planes.  tz is chosen so the
+// level offers at least ~1024 workgroups where it can.
+std::vector<Tile> sweep_tiles(const std::vector<BlockLevel> &hb, int ty) {
+    long long cols = 0;
+    int nzmax = 1;
+    for (const BlockLevel &g : hb) {
+        cols += (long long)((g.n0 + SW_TX - 1) / SW_TX) * ((g.n1 + ty - 1) / ty);
+        nzmax = std::max(nzmax, g.n2);
+    }
+    int tz = 32;
+    while (tz > 2 && cols * ((nzmax + tz - 1) / tz) < 1024) tz >>= 1;
+    std::vector<Tile> t;
+    for (int b = 0; b < (int)hb.size(); ++b) {
+        const BlockLevel &g = hb[b];
+        for (int z = 1; z <= g.n2; z += tz)
+            for (int y = 1; y <= g.n1; y += ty)
+                for (int x = 1; x <= g.n0; x += SW_TX)
+                    t.push_back(Tile{b, x, y, z, std::min(g.n2, z + tz - 1)});
     }
     return t;
 }
@@ -237,15 +275,15 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
         long long gh = 0;
         for (int i = 0; i < c.nblocks; ++i)
             gh += (long long)(dim_at(b[i], 0, l) + 2) * (dim_at(b[i], 1, l) + 2) * (dim_at(b[i], 2, l) + 2);
-        tot += gh * w * 2;                       // phi, rhs
-        if (c.variable_coef) tot += gh * (w * 3 + 8 * 4);  // coefficients + FP64 copies + density
+        tot += gh * w * 3;                                     // phi (two) and rhs
+        if (c.variable_coef) tot += gh * (w * 3 + 8 * 4);      // coefficients, FP64 copies, density
     }
     long long gh0 = 0, in0 = 0;
     for (int i = 0; i < c.nblocks; ++i) {
         gh0 += (long long)(b[i].n[0] + 2) * (b[i].n[1] + 2) * (b[i].n[2] + 2);
         in0 += (long long)b[i].n[0] * b[i].n[1] * b[i].n[2];
     }
-    tot += gh0 * 8 * 3;   // p (two buffers) and f
+    tot += gh0 * 8 * 3;             // p (two buffers) and f
     tot += std::max(gh0, in0) * 8;  // I/O staging
     return tot;
 }
@@ -275,125 +313,208 @@ void tk_flush(mf3d_mg *mg) {
 }
 
 // ---------------------------------------------------------------- launches
+inline const int *done_ptr(mf3d_mg *mg) { return &mg->st->done; }
+
 template <typename T>
-void fill(mf3d_mg *mg, T *a, long long n) {
+void fill(mf3d_mg *mg, T *a, long long n, const int *done = nullptr) {
     if (n <= 0) return;
     long long blocks = std::min<long long>((n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, mg->stream, a, n, T(0));
-}
-
-template <typename T, bool VAR>
-void sweep(mf3d_mg *mg, int l) {
-    Level &L = mg->lv[l];
-    if (!L.n_rb) return;
-    const bool tk = tk_on(mg, MF3D_K_SMOOTH, l);
-    for (int parity = 1; parity >= 0; --parity) {
-        if (tk) tk_mark(mg);
-        hipLaunchKernelGGL((k_rb<T, VAR>), dim3(L.n_rb), dim3(64, kTileY), 0, mg->stream, (T *)L.phi,
-                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db,
-                           L.t_rb, parity);
-        if (tk) tk_mark(mg);
-    }
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, mg->stream, a, n, T(0), done);
 }
 
 template <typename T>
-void gather(mf3d_mg *mg, int l, bool neumann_only) {
+void gather(mf3d_mg *mg, int l, const int *done) {
     Level &L = mg->lv[l];
-    const int nr = neumann_only ? L.n_neu : L.n_full;
-    const int mc = neumann_only ? L.max_neu : L.max_full;
-    if (!nr) return;
-    const int gx = std::max(1, std::min(64, (mc + 255) / 256));
-    hipLaunchKernelGGL(k_gather<T>, dim3(gx, nr), dim3(256), 0, mg->stream, (T *)L.phi, L.db,
-                       neumann_only ? L.r_neu : L.r_full);
+    if (!L.n_full) return;
+    const int gx = std::max(1, std::min(64, (L.max_full + 255) / 256));
+    hipLaunchKernelGGL(k_gather<T>, dim3(gx, L.n_full), dim3(256), 0, mg->stream, (T *)L.phi[L.cur], L.db,
+                       L.r_full, done);
 }
 
+template <typename T, bool VAR, bool ZERO>
+void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
+    const T *pin = (const T *)L.phi[L.cur];
+    T *pout = (T *)L.phi[1 - L.cur];
+    if (L.sw_ty == 8)
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db, L.t_sw,
+                           done);
+    else
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db, L.t_sw,
+                           done);
+}
+
+// `sweeps` sweeps at level l; the first treats the potential as zero when
+// zero_first (after restriction, or at the start of a cycle from pp = 0).
 template <typename T, bool VAR>
-void relax(mf3d_mg *mg, int l, int sweeps) {
+void relax(mf3d_mg *mg, int l, int sweeps, bool zero_first, const int *done) {
+    Level &L = mg->lv[l];
+    if (sweeps == 0 && zero_first) {  // nothing will overwrite the zero the reference leaves
+        fill<T>(mg, (T *)L.phi[L.cur], L.cells_gh, done);
+        return;
+    }
     for (int s = 0; s < sweeps; ++s) {
-        sweep<T, VAR>(mg, l);
-        gather<T>(mg, l, false);
+        const bool tk = tk_on(mg, MF3D_K_SMOOTH, l);
+        if (tk) tk_mark(mg);
+        if (zero_first && s == 0) sweep_launch<T, VAR, true>(mg, L, done);
+        else sweep_launch<T, VAR, false>(mg, L, done);
+        if (tk) tk_mark(mg);
+        L.cur = 1 - L.cur;
+        gather<T>(mg, l, done);
     }
 }
 
 template <typename T, bool VAR>
-void restrict_to(mf3d_mg *mg, int lc) {
+void restrict_to(mf3d_mg *mg, int lc, const int *done) {
     Level &C = mg->lv[lc], &F = mg->lv[lc - 1];
-    fill<T>(mg, (T *)C.phi, C.cells_gh);
     const bool tk = tk_on(mg, MF3D_K_RESTRICT, lc - 1);
     if (tk) tk_mark(mg);
     hipLaunchKernelGGL((k_restrict<T, VAR>), dim3(C.n_in), dim3(kTileX, kTileY), 0, mg->stream,
-                       (const T *)F.phi, (const T *)F.rhs, (const T *)F.cx, (const T *)F.cy,
-                       (const T *)F.cz, F.db, (T *)C.rhs, C.db, C.t_in);
+                       (const T *)F.phi[F.cur], (const T *)F.rhs, (const T *)F.cx, (const T *)F.cy,
+                       (const T *)F.cz, F.db, (T *)C.rhs, C.db, C.t_in, done);
     if (tk) tk_mark(mg);
 }
 
 template <typename T>
-void prolong_to(mf3d_mg *mg, int lf) {
+void prolong_to(mf3d_mg *mg, int lf, const int *done) {
     Level &C = mg->lv[lf + 1], &F = mg->lv[lf];
     const bool tk = tk_on(mg, MF3D_K_PROLONG, lf);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL(k_prolong<T>, dim3(F.n_gh), dim3(kTileX, kTileY), 0, mg->stream, (T *)F.phi,
-                       F.db, (const T *)C.phi, C.db, F.t_gh);
+    hipLaunchKernelGGL(k_prolong<T>, dim3(F.n_row), dim3(kTileX, kTileY), 0, mg->stream,
+                       (const T *)F.phi[F.cur], (T *)F.phi[1 - F.cur], F.db, (const T *)C.phi[C.cur], C.db,
+                       F.t_row, done);
     if (tk) tk_mark(mg);
-    gather<T>(mg, lf, true);
+    F.cur = 1 - F.cur;
 }
 
+// Levels lt..L-1 in one launch (k_tail); level lt's RHS is in place and its
+// potential starts from zero (or from phi_in when given).
 template <typename T, bool VAR>
-void vcycle_dev(mf3d_mg *mg, bool zero_start) {
-    const int L = mg->cfg.levels;
-    if (zero_start) fill<T>(mg, (T *)mg->lv[0].phi, mg->lv[0].cells_gh);
-    relax<T, VAR>(mg, 0, mg->cfg.nu_pre);
-    if (L > 1) {
-        for (int l = 1; l < L; ++l) {
-            restrict_to<T, VAR>(mg, l);
-            relax<T, VAR>(mg, l, mg->cfg.nu_pre + (l == L - 1 ? mg->cfg.nu_post : 0));
-        }
-        for (int l = L - 2; l >= 1; --l) {
-            prolong_to<T>(mg, l);
-            relax<T, VAR>(mg, l, mg->cfg.nu_post);
-        }
-        prolong_to<T>(mg, 0);
+void tail(mf3d_mg *mg, bool from_zero, const int *done) {
+    const int lt = mg->lt, L = mg->cfg.levels;
+    Level &V = mg->lv[lt];
+    TailArgs<T> co{};
+    for (int l = lt; l < L; ++l) {
+        co.cx[l - lt] = (const T *)mg->lv[l].cx;
+        co.cy[l - lt] = (const T *)mg->lv[l].cy;
+        co.cz[l - lt] = (const T *)mg->lv[l].cz;
     }
-    relax<T, VAR>(mg, 0, mg->cfg.nu_post);
+    hipLaunchKernelGGL((k_tail<T, VAR>), dim3(mg->cfg.nblocks), dim3(TAIL_NT), mg->tail_lds, mg->stream, mg->d_tail,
+                       mg->cfg.nblocks, L - lt, (const T *)V.rhs, from_zero ? nullptr : (const T *)V.phi[V.cur],
+                       (T *)V.phi[1 - V.cur], co, mg->cfg.nu_pre, mg->cfg.nu_post, done);
+    V.cur = 1 - V.cur;
 }
 
 template <typename T, bool VAR>
-void outer_step(mf3d_mg *mg, int k) {
+void vcycle_dev(mf3d_mg *mg, bool zero_start, const int *done) {
+    const int L = mg->cfg.levels, lt = mg->lt;
+    if (lt == 0) {
+        tail<T, VAR>(mg, zero_start, done);
+        return;
+    }
+    relax<T, VAR>(mg, 0, mg->cfg.nu_pre, zero_start, done);
+    if (L > 1) {
+        const int ldown = std::min(lt, L);
+        for (int l = 1; l < ldown; ++l) {
+            restrict_to<T, VAR>(mg, l, done);
+            relax<T, VAR>(mg, l, mg->cfg.nu_pre + (l == L - 1 ? mg->cfg.nu_post : 0), true, done);
+        }
+        if (lt < L) {
+            restrict_to<T, VAR>(mg, lt, done);
+            tail<T, VAR>(mg, true, done);
+        }
+        for (int l = ldown - 1; l >= 0; --l) {
+            if (l == L - 1) continue;
+            prolong_to<T>(mg, l, done);
+            if (l > 0) relax<T, VAR>(mg, l, mg->cfg.nu_post, false, done);
+        }
+    }
+    relax<T, VAR>(mg, 0, mg->cfg.nu_post, false, done);
+}
+
+template <typename T, bool VAR>
+void outer_step(mf3d_mg *mg, int from, const int *done) {
     Level &L0 = mg->lv[0];
     const double *cx = VAR ? L0.dcx : nullptr, *cy = VAR ? L0.dcy : nullptr, *cz = VAR ? L0.dcz : nullptr;
     const bool tk = tk_on(mg, MF3D_K_OUTER, 0);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL((k_outer<T, VAR>), dim3(L0.n_gh), dim3(kTileX, kTileY), 0, mg->stream, mg->p_a,
-                       mg->p_b, (const T *)L0.phi, mg->f_g, (T *)L0.rhs, cx, cy, cz, L0.db, L0.t_gh,
-                       mg->partials);
+    hipLaunchKernelGGL((k_outer_lds<T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, mg->pbuf[from],
+                       mg->pbuf[1 - from], (const T *)L0.phi[L0.cur], mg->f_g, (T *)L0.rhs, cx, cy, cz, L0.db,
+                       mg->t_out, mg->partials, done);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, L0.n_gh, mg->st,
-                       mg->d_hist, mg->d_rmax, k, mg->cfg.eps, mg->cfg.min_cycles);
-    std::swap(mg->p_a, mg->p_b);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts, mg->st,
+                       mg->d_hist, mg->d_rmax, mg->cfg.eps, mg->cfg.min_cycles);
+}
+
+// Record one cycle reading pbuf[g] into a graph.
+template <typename T, bool VAR>
+int capture_cycles(mf3d_mg *mg) {
+    for (int g = 0; g < 2; ++g) {
+        for (Level &V : mg->lv) V.cur = 0;
+        HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
+        vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
+        outer_step<T, VAR>(mg, g, done_ptr(mg));
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamEndCapture(mg->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&mg->cyc[g], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            mg->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            return MF3D_EDEVICE;
+        }
+    }
+    mg->graphs_ready = true;
+    return 0;
 }
 
 template <typename T, bool VAR>
-int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rmax_out,
-              int32_t *iters_out, double *hist) {
+int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rmax_out, int32_t *iters_out,
+              double *hist) {
     Level &L0 = mg->lv[0];
-    fill<double>(mg, mg->p_a, L0.cells_gh);
-    fill<double>(mg, mg->f_g, L0.cells_gh);
-    fill<T>(mg, (T *)L0.phi, L0.cells_gh);
-    hipLaunchKernelGGL(k_unpack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, f_packed,
-                       mg->f_g, L0.db, L0.t_in);
-    outer_step<T, VAR>(mg, 0);
-    int k = 0, done = 0;
+    const bool use_graph = mg->tk == MF3D_K_OFF;
+    if (use_graph && !mg->graphs_ready) {
+        int rc = capture_cycles<T, VAR>(mg);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st);
+    fill<double>(mg, mg->pbuf[0], L0.cells_gh);
+    fill<T>(mg, (T *)L0.phi[L0.cur], L0.cells_gh);
+    hipLaunchKernelGGL(k_unpack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, f_packed, mg->f_g,
+                       L0.db, L0.t_in);
+    // step 0: r_0 = f - A*0 from pbuf[0] into pbuf[1]
+    outer_step<T, VAR>(mg, 0, nullptr);
+    const int maxcy = mg->cfg.maxcy;
+    int launched = 0;
     SolveState hs{};
-    for (k = 1; k <= mg->cfg.maxcy; ++k) {
-        vcycle_dev<T, VAR>(mg, true);
-        outer_step<T, VAR>(mg, k);
+    hs.done = 0;
+    int chunk = mg->guess > 0 ? mg->guess : 4;
+    while (launched < maxcy) {
+        const int n = std::min(chunk, maxcy - launched);
+        for (int c = 0; c < n; ++c) {
+            const int k = launched + c + 1;  // cycle k reads pbuf[k % 2]
+            if (use_graph) {
+                HIPCHK(hipGraphLaunch(mg->cyc[k % 2], mg->stream));
+            } else {
+                for (Level &V : mg->lv) V.cur = 0;
+                vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
+                outer_step<T, VAR>(mg, k % 2, done_ptr(mg));
+            }
+        }
+        launched += n;
         HIPCHK(hipMemcpyAsync(&hs, mg->st, sizeof(SolveState), hipMemcpyDeviceToHost, mg->stream));
         HIPCHK(hipStreamSynchronize(mg->stream));
         tk_flush(mg);
-        if (hs.done) { done = 1; break; }
+        if (hs.done) break;
+        chunk = 1;
     }
-    if (k > mg->cfg.maxcy) k = mg->cfg.maxcy;
-    hipLaunchKernelGGL(k_pack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->p_a,
+    const int k = hs.last;  // cycles completed; p_k sits in pbuf[(k + 1) % 2]
+    if (k < 0) {
+        mg->err = "no outer step recorded";
+        return MF3D_EDEVICE;
+    }
+    mg->guess = k;
+    hipLaunchKernelGGL(k_pack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->pbuf[(k + 1) % 2],
                        p_packed, L0.db, L0.t_in);
     std::vector<double> hh(k + 1), rm(k + 1);
     HIPCHK(hipMemcpyAsync(hh.data(), mg->d_hist, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, mg->stream));
@@ -408,16 +529,11 @@ int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rma
         mg->err = "BIG RMAX (rmax > 100): diverged";
         return MF3D_DIVERGED;
     }
-    return done ? MF3D_OK : MF3D_NOT_CONVERGED;
+    return hs.done ? MF3D_OK : MF3D_NOT_CONVERGED;
 }
 
-template <typename T, bool VAR>
-int vcycle_host_t(mf3d_mg *mg) {
-    vcycle_dev<T, VAR>(mg, false);
-    return 0;
-}
-
-// Host arrays in the caller's layout <-> packed staging.
+// Host arrays in the caller's layout <-> packed staging (ring = ghost
+// layers kept around the interior).
 void host_to_packed(const mf3d_mg *mg, const double *const *src, double *dst, int ring) {
     long long o = 0;
     for (int b = 0; b < mg->cfg.nblocks; ++b) {
@@ -428,7 +544,7 @@ void host_to_packed(const mf3d_mg *mg, const double *const *src, double *dst, in
         const int s = B.ghost - ring;
         for (int k = 0; k < e2; ++k)
             for (int j = 0; j < e1; ++j) {
-                const double *row = src[b] + (size_t)(s) + (size_t)ld0 * ((size_t)(j + s) + (size_t)ld1 * (k + s));
+                const double *row = src[b] + (size_t)s + (size_t)ld0 * ((size_t)(j + s) + (size_t)ld1 * (k + s));
                 std::memcpy(dst + o, row, sizeof(double) * e0);
                 o += e0;
             }
@@ -445,26 +561,73 @@ void packed_to_host(const mf3d_mg *mg, const double *src, double *const *dst, in
         const int s = B.ghost - ring;
         for (int k = 0; k < e2; ++k)
             for (int j = 0; j < e1; ++j) {
-                double *row = dst[b] + (size_t)(s) + (size_t)ld0 * ((size_t)(j + s) + (size_t)ld1 * (k + s));
+                double *row = dst[b] + (size_t)s + (size_t)ld0 * ((size_t)(j + s) + (size_t)ld1 * (k + s));
                 std::memcpy(row, src + o, sizeof(double) * e0);
                 o += e0;
             }
     }
 }
 
-// Ghosted-packed (n+2)^3 per block <-> a level-0 arena (same shape, so a
-// block's packed ghosted array is exactly its arena slice).
-template <typename T>
-void ghosted_in(mf3d_mg *mg, const double *packed_dev, T *arena) {
-    Level &L0 = mg->lv[0];
-    hipLaunchKernelGGL((k_convert<T, double>), dim3((unsigned)std::min<long long>((L0.cells_gh + 255) / 256, 16384)),
-                       dim3(256), 0, mg->stream, arena, packed_dev, L0.cells_gh);
+template <typename TD, typename TS>
+void convert(mf3d_mg *mg, TD *dst, const TS *src, long long n) {
+    const unsigned gb = (unsigned)std::min<long long>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_convert<TD, TS>), dim3(gb), dim3(256), 0, mg->stream, dst, src, n);
 }
-template <typename T>
-void ghosted_out(mf3d_mg *mg, const T *arena, double *packed_dev) {
+
+// mgkcyc: the caller's pp (one ghost layer, same shape as the level-0 arena)
+// in and out, rhs into the level-0 RHS.
+template <typename T, bool VAR>
+int vcycle_host_t(mf3d_mg *mg) {
     Level &L0 = mg->lv[0];
-    hipLaunchKernelGGL((k_convert<double, T>), dim3((unsigned)std::min<long long>((L0.cells_gh + 255) / 256, 16384)),
-                       dim3(256), 0, mg->stream, packed_dev, arena, L0.cells_gh);
+    for (Level &V : mg->lv) V.cur = 0;
+    convert<T, double>(mg, (T *)L0.phi[0], mg->d_io, L0.cells_gh);
+    vcycle_dev<T, VAR>(mg, false, nullptr);
+    convert<double, T>(mg, mg->d_io, (const T *)L0.phi[L0.cur], L0.cells_gh);
+    return 0;
+}
+
+// The coarse tail runs in LDS when every block stands alone (no face
+// neighbour, no periodic partner) and the levels from lt down fit.
+constexpr size_t kTailLdsMax = 150 * 1024;
+
+int choose_tail(mf3d_mg *mg) {
+    const int L = mg->cfg.levels, nb = mg->cfg.nblocks;
+    mg->lt = L;
+    const char *env = std::getenv("MF3D_TAIL");
+    if (env && env[0] == '0') return 0;
+    for (const mf3d_block_desc &B : mg->blk)
+        for (int f = 0; f < 6; ++f)
+            if (B.nbr[f] >= 0 || B.per[f] >= 0) return 0;
+    const size_t w = inner_bytes(mg->cfg);
+    for (int lt = 0; lt < L; ++lt) {
+        size_t need = 0;
+        for (const mf3d_block_desc &B : mg->blk) {
+            size_t s = 0;
+            for (int l = lt; l < L; ++l)
+                s += 2 * w * (size_t)(dim_at(B, 0, l) + 2) * (dim_at(B, 1, l) + 2) * (dim_at(B, 2, l) + 2);
+            need = std::max(need, s);
+        }
+        if (need <= kTailLdsMax) {
+            mg->lt = lt;
+            mg->tail_lds = need;
+            break;
+        }
+    }
+    if (mg->lt == L) return 0;
+    // allow the tail's dynamic LDS beyond the default 64 KiB
+    const hipFuncAttribute attr = hipFuncAttributeMaxDynamicSharedMemorySize;
+    const int bytes = (int)mg->tail_lds;
+    if (hipFuncSetAttribute((const void *)k_tail<double, false>, attr, bytes) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_tail<double, true>, attr, bytes) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_tail<float, false>, attr, bytes) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_tail<float, true>, attr, bytes) != hipSuccess) {
+        mg->err = "hipFuncSetAttribute (tail LDS) failed";
+        return MF3D_EDEVICE;
+    }
+    std::vector<BlockLevel> tl;
+    for (int l = mg->lt; l < L; ++l)
+        for (int b = 0; b < nb; ++b) tl.push_back(mg->lv[l].hb[b]);
+    return upload(mg, &mg->d_tail, tl);
 }
 
 int build(mf3d_mg *mg) {
@@ -475,14 +638,18 @@ int build(mf3d_mg *mg) {
     mg->lv.resize(L);
     mg->cells = 0;
     for (int b = 0; b < nb; ++b) mg->cells += (long long)mg->blk[b].n[0] * mg->blk[b].n[1] * mg->blk[b].n[2];
+    int rc;
     for (int l = 0; l < L; ++l) {
         Level &V = mg->lv[l];
         long long off = 0, poff = 0;
         V.hb.resize(nb);
+        int n0max = 0, n1max = 0;
         for (int b = 0; b < nb; ++b) {
             const mf3d_block_desc &B = mg->blk[b];
             BlockLevel &g = V.hb[b];
             g.n0 = dim_at(B, 0, l); g.n1 = dim_at(B, 1, l); g.n2 = dim_at(B, 2, l);
+            n0max = std::max(n0max, g.n0);
+            n1max = std::max(n1max, g.n1);
             g.neu = 0;
             for (int f = 0; f < 6; ++f)
                 if (is_neumann(B, f)) g.neu |= (1 << f);
@@ -506,18 +673,17 @@ int build(mf3d_mg *mg) {
             g.gz = B.h[0] * B.h[1] / B.h[2];
         }
         V.cells_gh = off;
-        int rc = upload(mg, &V.db, V.hb);
-        if (rc) return rc;
-        auto trb = build_tiles(V.hb, kRbTileX, false), tin = build_tiles(V.hb, kTileX, false),
-             tgh = build_tiles(V.hb, kTileX, true);
-        V.n_rb = (int)trb.size(); V.n_in = (int)tin.size(); V.n_gh = (int)tgh.size();
-        if ((rc = upload(mg, &V.t_rb, trb)) || (rc = upload(mg, &V.t_in, tin)) || (rc = upload(mg, &V.t_gh, tgh)))
+        V.sw_ty = (n1max >= 8) ? 8 : 4;
+        if ((rc = upload(mg, &V.db, V.hb))) return rc;
+        auto tsw = sweep_tiles(V.hb, V.sw_ty), tin = cell_tiles(V.hb), trow = row_tiles(V.hb);
+        V.n_sw = (int)tsw.size(); V.n_in = (int)tin.size(); V.n_row = (int)trow.size();
+        if ((rc = upload(mg, &V.t_sw, tsw)) || (rc = upload(mg, &V.t_in, tin)) || (rc = upload(mg, &V.t_row, trow)))
             return rc;
-        auto rf = build_regions(mg->blk, l, false, &V.max_full);
-        auto rn = build_regions(mg->blk, l, true, &V.max_neu);
-        V.n_full = (int)rf.size(); V.n_neu = (int)rn.size();
-        if ((rc = upload(mg, &V.r_full, rf)) || (rc = upload(mg, &V.r_neu, rn))) return rc;
-        if ((rc = dev_alloc(mg, &V.phi, V.cells_gh * w)) || (rc = dev_alloc(mg, &V.rhs, V.cells_gh * w)))
+        auto rf = build_regions(mg->blk, l, &V.max_full);
+        V.n_full = (int)rf.size();
+        if ((rc = upload(mg, &V.r_full, rf))) return rc;
+        if ((rc = dev_alloc(mg, &V.phi[0], V.cells_gh * w)) || (rc = dev_alloc(mg, &V.phi[1], V.cells_gh * w)) ||
+            (rc = dev_alloc(mg, &V.rhs, V.cells_gh * w)))
             return rc;
         if (c.variable_coef) {
             if ((rc = dev_alloc(mg, (void **)&V.dcx, V.cells_gh * 8)) ||
@@ -535,12 +701,17 @@ int build(mf3d_mg *mg) {
         }
     }
     Level &L0 = mg->lv[0];
-    int rc;
-    if ((rc = dev_alloc(mg, (void **)&mg->p_a, L0.cells_gh * 8)) ||
-        (rc = dev_alloc(mg, (void **)&mg->p_b, L0.cells_gh * 8)) ||
+    {
+        auto tout = sweep_tiles(L0.hb, 8);
+        mg->n_out = (int)tout.size();
+        if ((rc = upload(mg, &mg->t_out, tout))) return rc;
+    }
+    if ((rc = choose_tail(mg))) return rc;
+    if ((rc = dev_alloc(mg, (void **)&mg->pbuf[0], L0.cells_gh * 8)) ||
+        (rc = dev_alloc(mg, (void **)&mg->pbuf[1], L0.cells_gh * 8)) ||
         (rc = dev_alloc(mg, (void **)&mg->f_g, L0.cells_gh * 8)))
         return rc;
-    mg->nparts = L0.n_gh;
+    mg->nparts = mg->n_out;
     if ((rc = dev_alloc(mg, (void **)&mg->partials, sizeof(double) * 2 * (size_t)mg->nparts)) ||
         (rc = dev_alloc(mg, (void **)&mg->st, sizeof(SolveState))) ||
         (rc = dev_alloc(mg, (void **)&mg->d_hist, sizeof(double) * (c.maxcy + 1))) ||
@@ -549,16 +720,24 @@ int build(mf3d_mg *mg) {
     mg->io_len = std::max(L0.cells_gh, mg->cells);
     if ((rc = dev_alloc(mg, (void **)&mg->d_io, sizeof(double) * mg->io_len))) return rc;
     HIPCHK(hipHostMalloc((void **)&mg->h_io, sizeof(double) * mg->io_len, hipHostMallocDefault));
-    // level fields start at zero (coeff zeroes phi, mgsolver.for:60-67)
+    // every field starts at zero (coeff zeroes phi, mgsolver.for:60-67)
     for (int l = 0; l < L; ++l) {
         Level &V = mg->lv[l];
-        if (mixed) { fill<float>(mg, (float *)V.phi, V.cells_gh); fill<float>(mg, (float *)V.rhs, V.cells_gh); }
-        else { fill<double>(mg, (double *)V.phi, V.cells_gh); fill<double>(mg, (double *)V.rhs, V.cells_gh); }
+        if (mixed) {
+            fill<float>(mg, (float *)V.phi[0], V.cells_gh); fill<float>(mg, (float *)V.phi[1], V.cells_gh);
+            fill<float>(mg, (float *)V.rhs, V.cells_gh);
+        } else {
+            fill<double>(mg, (double *)V.phi[0], V.cells_gh); fill<double>(mg, (double *)V.phi[1], V.cells_gh);
+            fill<double>(mg, (double *)V.rhs, V.cells_gh);
+        }
         if (c.variable_coef) {
             fill<double>(mg, V.dcx, V.cells_gh); fill<double>(mg, V.dcy, V.cells_gh);
             fill<double>(mg, V.dcz, V.cells_gh); fill<double>(mg, V.dens, V.cells_gh);
         }
     }
+    fill<double>(mg, mg->pbuf[1], L0.cells_gh);
+    fill<double>(mg, mg->f_g, L0.cells_gh);
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st);
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
     return 0;
@@ -574,13 +753,12 @@ int build_lsm(mf3d_mg *mg) {
     }
     for (int l = 0; l < L; ++l) {
         Level &V = mg->lv[l];
-        hipLaunchKernelGGL(k_lsm_coef, dim3(V.n_gh), dim3(kTileX, kTileY), 0, mg->stream, V.dens, V.dcx, V.dcy,
-                           V.dcz, V.db, V.t_gh);
+        hipLaunchKernelGGL(k_lsm_coef, dim3(V.n_row), dim3(kTileX, kTileY), 0, mg->stream, V.dens, V.dcx, V.dcy,
+                           V.dcz, V.db, V.t_row);
         if (mg->cfg.precision == MF3D_MIXED) {
-            const unsigned gb = (unsigned)std::min<long long>((V.cells_gh + 255) / 256, 16384);
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(gb), dim3(256), 0, mg->stream, (float *)V.cx, V.dcx, V.cells_gh);
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(gb), dim3(256), 0, mg->stream, (float *)V.cy, V.dcy, V.cells_gh);
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(gb), dim3(256), 0, mg->stream, (float *)V.cz, V.dcz, V.cells_gh);
+            convert<float, double>(mg, (float *)V.cx, V.dcx, V.cells_gh);
+            convert<float, double>(mg, (float *)V.cy, V.dcy, V.cells_gh);
+            convert<float, double>(mg, (float *)V.cz, V.dcz, V.cells_gh);
         }
     }
     HIPCHK(hipStreamSynchronize(mg->stream));
@@ -592,6 +770,33 @@ int build_lsm(mf3d_mg *mg) {
 int set_device(mf3d_mg *mg) {
     HIPCHK(hipSetDevice(mg->cfg.device));
     return 0;
+}
+
+int check_ready(mf3d_mg *mg) {
+    if (mg->cfg.variable_coef && !mg->density_set) {
+        mg->err = "variable-coefficient handle needs mf3d_mg_set_density first";
+        return MF3D_EINVAL;
+    }
+    return 0;
+}
+
+int solve_any(mf3d_mg *mg, const double *f_dev, double *p_dev, double *rmax_out, int32_t *iters_out,
+              double *hist) {
+    const bool mixed = mg->cfg.precision == MF3D_MIXED, var = mg->cfg.variable_coef != 0;
+    if (mixed)
+        return var ? solve_dev<float, true>(mg, f_dev, p_dev, rmax_out, iters_out, hist)
+                   : solve_dev<float, false>(mg, f_dev, p_dev, rmax_out, iters_out, hist);
+    return var ? solve_dev<double, true>(mg, f_dev, p_dev, rmax_out, iters_out, hist)
+               : solve_dev<double, false>(mg, f_dev, p_dev, rmax_out, iters_out, hist);
+}
+
+void drop_graphs(mf3d_mg *mg) {
+    for (hipGraphExec_t &g : mg->cyc)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    mg->graphs_ready = false;
 }
 
 }  // namespace
@@ -615,28 +820,28 @@ int mf3d_mg_create(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, mf3d_m
     std::string v = validate(*cfg, blocks);
     mf3d_mg *mg = new mf3d_mg();
     mg->cfg = *cfg;
+    *out = mg;  // returned even on failure so the caller can read the message
     if (!v.empty()) {
         mg->err = v;
-        *out = mg;  // handle returned so the caller can read the message
         return MF3D_EINVAL;
     }
     mg->blk.assign(blocks, blocks + cfg->nblocks);
     int rc = set_device(mg);
-    if (rc) { *out = mg; return rc; }
+    if (rc) return rc;
     hipError_t e = hipStreamCreate(&mg->stream);
     if (e != hipSuccess) {
         mg->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
-        *out = mg;
         return MF3D_EDEVICE;
     }
-    rc = build(mg);
-    *out = mg;
-    return rc;
+    return build(mg);
 }
 
 int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks) {
     if (!mg || !dens_blocks) return MF3D_EINVAL;
-    if (!mg->cfg.variable_coef) { mg->err = "handle was created without variable_coef"; return MF3D_EINVAL; }
+    if (!mg->cfg.variable_coef) {
+        mg->err = "handle was created without variable_coef";
+        return MF3D_EINVAL;
+    }
     int rc = set_device(mg);
     if (rc) return rc;
     host_to_packed(mg, dens_blocks, mg->h_io, 0);
@@ -647,24 +852,18 @@ int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks) {
     return build_lsm(mg);
 }
 
-static int check_ready(mf3d_mg *mg) {
-    if (mg->cfg.variable_coef && !mg->density_set) {
-        mg->err = "variable-coefficient handle needs mf3d_mg_set_density first";
-        return MF3D_EINVAL;
-    }
-    return 0;
-}
-
 int mf3d_mg_vcycle(mf3d_mg *mg, const double *const *rhs_blocks, double *const *pp_blocks) {
     if (!mg || !rhs_blocks || !pp_blocks) return MF3D_EINVAL;
     int rc = check_ready(mg);
     if (rc) return rc;
     for (int b = 0; b < mg->cfg.nblocks; ++b)
-        if (mg->blk[b].ghost < 1) { mg->err = "vcycle needs caller arrays with ghost >= 1"; return MF3D_EINVAL; }
+        if (mg->blk[b].ghost < 1) {
+            mg->err = "vcycle needs caller arrays with ghost >= 1";
+            return MF3D_EINVAL;
+        }
     if ((rc = set_device(mg))) return rc;
     Level &L0 = mg->lv[0];
-    const bool mixed = mg->cfg.precision == MF3D_MIXED;
-    // rhs (interior)
+    const bool mixed = mg->cfg.precision == MF3D_MIXED, var = mg->cfg.variable_coef != 0;
     host_to_packed(mg, rhs_blocks, mg->h_io, 0);
     HIPCHK(hipMemcpyAsync(mg->d_io, mg->h_io, sizeof(double) * mg->cells, hipMemcpyHostToDevice, mg->stream));
     if (mixed)
@@ -673,34 +872,17 @@ int mf3d_mg_vcycle(mf3d_mg *mg, const double *const *rhs_blocks, double *const *
     else
         hipLaunchKernelGGL(k_unpack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->d_io,
                            (double *)L0.rhs, L0.db, L0.t_in);
-    // pp including one ghost layer == the level-0 arena shape
     HIPCHK(hipStreamSynchronize(mg->stream));
     host_to_packed(mg, pp_blocks, mg->h_io, 1);
     HIPCHK(hipMemcpyAsync(mg->d_io, mg->h_io, sizeof(double) * L0.cells_gh, hipMemcpyHostToDevice, mg->stream));
-    const bool var = mg->cfg.variable_coef != 0;
-    if (mixed) {
-        ghosted_in<float>(mg, mg->d_io, (float *)L0.phi);
-        if (var) vcycle_host_t<float, true>(mg); else vcycle_host_t<float, false>(mg);
-        ghosted_out<float>(mg, (const float *)L0.phi, mg->d_io);
-    } else {
-        ghosted_in<double>(mg, mg->d_io, (double *)L0.phi);
-        if (var) vcycle_host_t<double, true>(mg); else vcycle_host_t<double, false>(mg);
-        ghosted_out<double>(mg, (const double *)L0.phi, mg->d_io);
-    }
+    if (mixed) rc = var ? vcycle_host_t<float, true>(mg) : vcycle_host_t<float, false>(mg);
+    else rc = var ? vcycle_host_t<double, true>(mg) : vcycle_host_t<double, false>(mg);
+    if (rc) return rc;
     HIPCHK(hipMemcpyAsync(mg->h_io, mg->d_io, sizeof(double) * L0.cells_gh, hipMemcpyDeviceToHost, mg->stream));
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
     packed_to_host(mg, mg->h_io, pp_blocks, 1);
     return 0;
-}
-
-static int solve_any(mf3d_mg *mg, const double *f_dev, double *p_dev, double *rmax_out, int32_t *iters_out,
-                     double *hist) {
-    const bool mixed = mg->cfg.precision == MF3D_MIXED, var = mg->cfg.variable_coef != 0;
-    if (mixed) return var ? solve_dev<float, true>(mg, f_dev, p_dev, rmax_out, iters_out, hist)
-                          : solve_dev<float, false>(mg, f_dev, p_dev, rmax_out, iters_out, hist);
-    return var ? solve_dev<double, true>(mg, f_dev, p_dev, rmax_out, iters_out, hist)
-               : solve_dev<double, false>(mg, f_dev, p_dev, rmax_out, iters_out, hist);
 }
 
 int mf3d_mg_solve(mf3d_mg *mg, const double *const *f_blocks, double *const *p_blocks, double *rmax_out,
@@ -719,6 +901,7 @@ int mf3d_mg_solve(mf3d_mg *mg, const double *const *f_blocks, double *const *p_b
         HIPCHK(hipStreamSynchronize(mg->stream));
         packed_to_host(mg, mg->h_io, p_blocks, 0);
     }
+    (void)hipStreamSynchronize(mg->stream);
     mg->allocs.erase(std::find(mg->allocs.begin(), mg->allocs.end(), (void *)f_dev));
     mg->bytes -= (long long)(((sizeof(double) * mg->cells) + 255) & ~size_t(255));
     (void)hipFree(f_dev);
@@ -768,6 +951,7 @@ const char *mf3d_mg_last_error(const mf3d_mg *mg) {
 void mf3d_mg_destroy(mf3d_mg *mg) {
     if (!mg) return;
     if (mg->stream) (void)hipStreamSynchronize(mg->stream);
+    drop_graphs(mg);
     for (void *p : mg->allocs) (void)hipFree(p);
     if (mg->h_io) (void)hipHostFree(mg->h_io);
     for (hipEvent_t e : mg->ev) (void)hipEventDestroy(e);

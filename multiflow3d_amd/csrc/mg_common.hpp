@@ -1,0 +1,132 @@
+// mg_common.hpp -- data layout and shared device helpers of the gfx950
+// pressure-Poisson multigrid kernels.
+//
+// Field layout in HBM: per level, one arena per field holding every block's
+// ghosted array ((n0+2)*(n1+2)*(n2+2) values, i fastest), blocks one after
+// another.  A BlockLevel record gives a block's extent, strides and arena
+// offset at one level.  Kernels walk a per-level tile table, so one launch
+// covers every block of a batch whatever their extents.  The potential has
+// two arenas per level (ping-pong): a sweep reads one and writes the other.
+//
+// Arithmetic follows the reference operand by operand (mgsolver.for:563-569
+// smoother, :702-709 residual, :743-747 restriction, :833-887 prolongation);
+// the library is compiled with -ffp-contract=off so the FP64 path rounds
+// exactly as the reference does.
+//
+// Every kernel that runs inside a captured cycle takes `done`, the device
+// flag raised once the solve has converged: later cycles of a speculative
+// launch then return at once.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mf3d {
+
+enum : int { kNeuW = 1, kNeuE = 2, kNeuS = 4, kNeuN = 8, kNeuB = 16, kNeuT = 32 };
+
+struct BlockLevel {
+    int n0, n1, n2;  // interior cells
+    int neu;         // faces that are physical Neumann walls (kNeu* bits)
+    long long sy;    // ghosted stride of j
+    long long sz;    // ghosted stride of k
+    long long off;   // element offset of this block in the level arena
+    long long poff;  // offset of this block in interior-packed vectors
+    double c0, c1, c2;  // 1/h_l^2 per axis (constant coefficients)
+    float f0, f1, f2;   // the same in FP32
+    int lsm;            // rmax measured without the cell volume
+    double h0, h1, h2;  // finest dx, dy, dz
+    double gx, gy, gz;  // LSM face factors dy*dz/dx, dx*dz/dy, dx*dy/dz (finest h)
+};
+
+// A unit of work: block b, x from x0, rows from y0, planes z0..z1.
+struct Tile {
+    int b, x0, y0, z0, z1;
+};
+
+// One ghost region of one block: every destination cell in the box takes
+// the value of a source cell; per axis the source coordinate is either the
+// destination coordinate (map < 0) or the constant map.
+struct Region {
+    int dst_b, src_b;
+    int lo[3];
+    int cnt[3];
+    int map[3];
+};
+
+struct SolveState {
+    double fnorm;
+    int done;   // converged: later cycle kernels return at once
+    int k;      // index of the next outer step
+    int last;   // last outer step recorded
+    int pad;
+};
+
+__device__ __forceinline__ long long gidx(const BlockLevel &g, int i, int j, int k) {
+    return g.off + (long long)i + g.sy * j + g.sz * k;
+}
+
+template <typename T> __device__ __forceinline__ T cconst(const BlockLevel &g, int d);
+template <> __device__ __forceinline__ double cconst<double>(const BlockLevel &g, int d) {
+    return d == 0 ? g.c0 : (d == 1 ? g.c1 : g.c2);
+}
+template <> __device__ __forceinline__ float cconst<float>(const BlockLevel &g, int d) {
+    return d == 0 ? g.f0 : (d == 1 ? g.f1 : g.f2);
+}
+
+// The six neighbour coefficients of cell (i,j,k) and ap (mgsolver.for:184-208).
+// Constant form: 1/h^2 with a zero on a physical Neumann face.  Variable
+// form: face arrays cx, cy, cz where cx at (i,j,k) is aw of cell i and cx at
+// (n0+1,j,k) is ae of the last cell (aw(i+1) == ae(i) bitwise inside a block).
+template <typename T, bool VAR>
+__device__ __forceinline__ void coefs(const BlockLevel &g, const T *__restrict__ cx,
+                                      const T *__restrict__ cy, const T *__restrict__ cz,
+                                      long long q, int i, int j, int k, T &aw, T &ae, T &as,
+                                      T &an, T &ab, T &at, T &ap) {
+    if (VAR) {
+        aw = cx[q];
+        ae = cx[q + 1];
+        as = cy[q];
+        an = cy[q + g.sy];
+        ab = cz[q];
+        at = cz[q + g.sz];
+    } else {
+        const T a = cconst<T>(g, 0), b = cconst<T>(g, 1), c = cconst<T>(g, 2);
+        aw = (i == 1 && (g.neu & kNeuW)) ? T(0) : a;
+        ae = (i == g.n0 && (g.neu & kNeuE)) ? T(0) : a;
+        as = (j == 1 && (g.neu & kNeuS)) ? T(0) : b;
+        an = (j == g.n1 && (g.neu & kNeuN)) ? T(0) : b;
+        ab = (k == 1 && (g.neu & kNeuB)) ? T(0) : c;
+        at = (k == g.n2 && (g.neu & kNeuT)) ? T(0) : c;
+    }
+    T s = aw + ae;
+    s = s + as;
+    s = s + an;
+    s = s + ab;
+    s = s + at;
+    ap = -s;   // -1.0*(...) of the reference: negation is exact
+}
+
+// Fixed-shape tree reduction of (sum, max) over a workgroup of NT threads
+// (a power of two); the order never changes, so results are bitwise
+// reproducible run to run.
+template <int NT>
+__device__ __forceinline__ void block_sum_max(double s, double m, double *out) {
+    __shared__ double ss[NT], sm[NT];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    ss[t] = s;
+    sm[t] = m;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] = ss[t] + ss[t + w];
+            sm[t] = fmax(sm[t], sm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = ss[0];
+        out[1] = sm[0];
+    }
+}
+
+}  // namespace mf3d

@@ -1,0 +1,182 @@
+// mg_outer.hpp -- FP64 outer step of the defect-correction loop and the
+// residual norms.
+#pragma once
+#include "mg_common.hpp"
+#include "mg_sweep.hpp"
+
+namespace mf3d {
+
+// FP64 outer step (newsolv_mg.for:38-54 in Poisson form):
+// p_new = p_old + e over every ghosted cell (both operands carry mapped
+// ghosts, so p_new does too), and on interior cells r = f - A p_new.  r
+// becomes the next cycle's level-0 RHS (rounded to the inner precision TE);
+// per-workgroup partial sums of r^2 and max |r| dV go to `partials`.
+//
+// Same tiling as the sweep: 64 x TY columns marching over planes, with
+// pn = p_old + e formed once per cell into a 4-plane LDS ring (1-cell halo
+// in x and y).  Each workgroup writes p_new for its own cells plus the ghost
+// cells next to them on the block boundary.  Plane c+2 of p_old and e and
+// the next plane's f are loaded while plane c is computed.
+template <typename TE, bool VAR, int TY>
+__global__ __launch_bounds__(SW_TX *TY) void k_outer_lds(
+    const double *__restrict__ pold, double *__restrict__ pnew, const TE *__restrict__ e,
+    const double *__restrict__ f, TE *__restrict__ rhs0, const double *__restrict__ cx,
+    const double *__restrict__ cy, const double *__restrict__ cz,
+    const BlockLevel *__restrict__ bls, const Tile *__restrict__ tiles,
+    double *__restrict__ partials, const int *__restrict__ done) {
+    if (done && *done) return;
+    constexpr int LX = SW_TX + 2, LY = TY + 2, NS = 4, NT = SW_TX * TY, LXY = LX * LY;
+    constexpr int KL = (LXY + NT - 1) / NT;
+    __shared__ double sp[NS][LXY];
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b];
+    const int tid = threadIdx.x + SW_TX * threadIdx.y;
+    const int xb = t.x0 - 1, yb = t.y0 - 1;
+    const int xhi = t.x0 + SW_TX - 1 < g.n0 ? t.x0 + SW_TX - 1 : g.n0;
+    const int yhi = t.y0 + TY - 1 < g.n1 ? t.y0 + TY - 1 : g.n1;
+    const int wx0 = t.x0 == 1 ? 0 : t.x0, wx1 = xhi == g.n0 ? g.n0 + 1 : xhi;
+    const int wy0 = t.y0 == 1 ? 0 : t.y0, wy1 = yhi == g.n1 ? g.n1 + 1 : yhi;
+    const int wz0 = t.z0 == 1 ? 0 : t.z0, wz1 = t.z1 == g.n2 ? g.n2 + 1 : t.z1;
+
+    long long lofs[KL];
+    bool lok[KL], lown[KL];
+#pragma unroll
+    for (int k = 0; k < KL; ++k) {
+        const int p = tid + k * NT;
+        const int ly = p / LX, lx = p - ly * LX;
+        const int x = xb + lx, y = yb + ly;
+        lok[k] = p < LXY && x >= 0 && x <= g.n0 + 1 && y >= 0 && y <= g.n1 + 1;
+        lown[k] = lok[k] && x >= wx0 && x <= wx1 && y >= wy0 && y <= wy1;
+        lofs[k] = g.off + (long long)x + g.sy * y;
+    }
+    // unconditional loads from clamped addresses, masked afterwards, in a
+    // 3-deep pipeline of register sets (see k_rbs)
+    long long lsrc[KL];
+#pragma unroll
+    for (int k = 0; k < KL; ++k) lsrc[k] = lok[k] ? lofs[k] : g.off;
+    const int bx = t.x0 + threadIdx.x, by = t.y0 + threadIdx.y;
+    const bool has_col = bx <= g.n0 && by <= g.n1;
+    struct Regs {
+        double pp[KL];
+        TE pe[KL];
+        double ff;
+    };
+    auto load_regs = [&](Regs &R, int c) {  // plane c+1 of p_old and e, f on plane c
+        const long long zo = g.sz * (long long)(c + 1 < t.z1 + 1 ? c + 1 : t.z1 + 1);
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const double a = pold[lsrc[k] + zo];
+            const TE b = e[lsrc[k] + zo];
+            R.pp[k] = lok[k] ? a : 0.0;
+            R.pe[k] = lok[k] ? b : TE(0);
+        }
+        const bool ok = has_col && c >= t.z0 && c <= t.z1;
+        const double v = f[ok ? gidx(g, bx, by, c) : g.off];
+        R.ff = ok ? v : 0.0;
+    };
+    auto store_plane = [&](int c, const double *pp, const TE *pe) {
+        double *P = sp[c % NS];
+        const bool zown = c >= wz0 && c <= wz1;
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            if (tid + k * NT >= LXY) continue;
+            const double v = pp[k] + (double)pe[k];
+            if (zown && lown[k]) pnew[lofs[k] + g.sz * c] = v;
+            P[tid + k * NT] = v;
+        }
+    };
+    double r2 = 0.0, ra = 0.0;
+    {
+        Regs R;
+        load_regs(R, t.z0 - 2);
+        store_plane(t.z0 - 1, R.pp, R.pe);
+        load_regs(R, t.z0 - 1);
+        store_plane(t.z0, R.pp, R.pe);
+    }
+    auto step = [&](int c, Regs &R) {
+        store_plane(c + 1, R.pp, R.pe);
+        const double f_cur = R.ff;
+        load_regs(R, c + 3);
+        __syncthreads();
+        if (has_col) {
+            const double *P = sp[c % NS];
+            const double *Pm = sp[(c + NS - 1) % NS];
+            const double *Pp = sp[(c + 1) % NS];
+            const int l = (bx - xb) + LX * (by - yb);
+            const long long q = gidx(g, bx, by, c);
+            double aw, ae, as, an, ab, at, ap;
+            coefs<double, VAR>(g, cx, cy, cz, q, bx, by, c, aw, ae, as, an, ab, at, ap);
+            double s = aw * P[l - 1];
+            s = s + ae * P[l + 1];
+            s = s + as * P[l - LX];
+            s = s + an * P[l + LX];
+            s = s + ab * Pm[l];
+            s = s + at * Pp[l];
+            s = s + ap * P[l];
+            const double r = f_cur - s;
+            rhs0[q] = (TE)r;
+            r2 = r2 + r * r;
+            ra = fmax(ra, g.lsm ? fabs(r) : fabs(r * g.h0 * g.h1 * g.h2));
+        }
+    };
+    Regs R0, R1, R2;
+    load_regs(R0, t.z0);
+    load_regs(R1, t.z0 + 1);
+    load_regs(R2, t.z0 + 2);
+    int c = t.z0;
+    for (; c + 2 <= t.z1; c += 3) {
+        step(c, R0);
+        step(c + 1, R1);
+        step(c + 2, R2);
+    }
+    if (c <= t.z1) step(c, R0);
+    if (c + 1 <= t.z1) step(c + 1, R1);
+    block_sum_max<SW_TX * TY>(r2, ra, partials + 2 * (long long)blockIdx.x);
+}
+
+// Reduce the partials in a fixed order; record rho_k = ||r_k|| / ||f|| and
+// rmax_k for k = st->k; raise `done` when converged (k >= 1 and
+// rho_k <= tol and k >= min_cycles).
+__global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ partials, int nparts,
+                                                   SolveState *st, double *hist, double *rmaxh,
+                                                   double tol, int min_cycles) {
+    if (st->done) return;
+    __shared__ double ss[1024], sm[1024];
+    const int t = threadIdx.x;
+    double s = 0.0, m = 0.0;
+    for (int p = t; p < nparts; p += 1024) {
+        s = s + partials[2 * (long long)p];
+        m = fmax(m, partials[2 * (long long)p + 1]);
+    }
+    ss[t] = s;
+    sm[t] = m;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            ss[t] = ss[t] + ss[t + w];
+            sm[t] = fmax(sm[t], sm[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int k = st->k;
+        const double nr = sqrt(ss[0]);
+        if (k == 0) st->fnorm = nr > 0.0 ? nr : 1.0;
+        const double rho = nr / st->fnorm;
+        hist[k] = rho;
+        rmaxh[k] = sm[0];
+        st->last = k;
+        st->k = k + 1;
+        if (k >= 1 && rho <= tol && k >= min_cycles) st->done = 1;
+    }
+}
+
+__global__ void k_reset_state(SolveState *st) {
+    st->fnorm = 1.0;
+    st->done = 0;
+    st->k = 0;
+    st->last = -1;
+}
+
+
+}  // namespace mf3d

@@ -1,0 +1,130 @@
+// mg_setup.hpp -- coefficient hierarchy, fills and layout conversions.
+#pragma once
+#include "mg_common.hpp"
+
+namespace mf3d {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ a, long long n, T v,
+                                              const int *__restrict__ done) {
+    if (done && *done) return;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) a[t] = v;
+}
+
+// LSM density hierarchy (mgsolver.for:81-102): 8-cell average in the
+// reference's child order (0,0,0) (0,1,0) (1,0,0) (1,1,0) (0,0,1) (0,1,1)
+// (1,0,1) (1,1,1) as (di,dj,dk).
+__global__ __launch_bounds__(256) void k_dens_restrict(const double *__restrict__ df,
+                                                       const BlockLevel *__restrict__ blf,
+                                                       double *__restrict__ dc,
+                                                       const BlockLevel *__restrict__ blc,
+                                                       const Tile *__restrict__ tiles) {
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel gc = blc[t.b];
+    const int i = t.x0 + threadIdx.x, j = t.y0 + threadIdx.y, k = t.z0;
+    if (i > gc.n0 || j > gc.n1) return;
+    const BlockLevel gf = blf[t.b];
+    const long long q = gidx(gf, 2 * i - 1, 2 * j - 1, 2 * k - 1);
+    double s = df[q];
+    s = s + df[q + gf.sy];
+    s = s + df[q + 1];
+    s = s + df[q + 1 + gf.sy];
+    s = s + df[q + gf.sz];
+    s = s + df[q + gf.sy + gf.sz];
+    s = s + df[q + 1 + gf.sz];
+    s = s + df[q + 1 + gf.sy + gf.sz];
+    dc[gidx(gc, i, j, k)] = 0.125 * s;
+}
+
+// LSM face coefficients (mgsolver.for:120-203) on one level: cx at
+// (i,j,k), i = 1..n0, is aw of cell i = (dy*dz/dx)/rho_face with rho_face
+// the mean of the two cells (the cell itself at the block edge); cx at
+// i = n0+1 is ae of the last cell (one-sided).  A physical Neumann face
+// gets 0.  The finest dx, dy, dz are used on every level, as the
+// reference does.  Row tiles over the ghosted range.
+__global__ __launch_bounds__(256) void k_lsm_coef(const double *__restrict__ d,
+                                                  double *__restrict__ cx, double *__restrict__ cy,
+                                                  double *__restrict__ cz,
+                                                  const BlockLevel *__restrict__ bls,
+                                                  const Tile *__restrict__ tiles) {
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b];
+    const int j = t.y0 + threadIdx.y, k = t.z0;
+    if (j > g.n1 + 1) return;
+    for (int i = threadIdx.x; i <= g.n0 + 1; i += blockDim.x) {
+        const long long q = gidx(g, i, j, k);
+        const bool jin = j >= 1 && j <= g.n1, kin = k >= 1 && k <= g.n2, iin = i >= 1 && i <= g.n0;
+        if (jin && kin && i >= 1) {
+            double v;
+            if (i <= g.n0) {
+                const double d0 = d[q];
+                const double dm = (i == 1) ? d0 : 0.5 * (d0 + d[q - 1]);
+                v = (i == 1 && (g.neu & kNeuW)) ? 0.0 : g.gx / dm;
+            } else {
+                v = (g.neu & kNeuE) ? 0.0 : g.gx / d[q - 1];
+            }
+            cx[q] = v;
+        }
+        if (iin && kin && j >= 1) {
+            double v;
+            if (j <= g.n1) {
+                const double d0 = d[q];
+                const double dm = (j == 1) ? d0 : 0.5 * (d0 + d[q - g.sy]);
+                v = (j == 1 && (g.neu & kNeuS)) ? 0.0 : g.gy / dm;
+            } else {
+                v = (g.neu & kNeuN) ? 0.0 : g.gy / d[q - g.sy];
+            }
+            cy[q] = v;
+        }
+        if (iin && jin && k >= 1) {
+            double v;
+            if (k <= g.n2) {
+                const double d0 = d[q];
+                const double dm = (k == 1) ? d0 : 0.5 * (d0 + d[q - g.sz]);
+                v = (k == 1 && (g.neu & kNeuB)) ? 0.0 : g.gz / dm;
+            } else {
+                v = (g.neu & kNeuT) ? 0.0 : g.gz / d[q - g.sz];
+            }
+            cz[q] = v;
+        }
+    }
+}
+
+template <typename TD, typename TS>
+__global__ __launch_bounds__(256) void k_convert(TD *__restrict__ dst, const TS *__restrict__ src,
+                                                 long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) dst[t] = (TD)src[t];
+}
+
+// Interior-packed vector <-> ghosted arena.
+template <typename T>
+__global__ __launch_bounds__(256) void k_unpack(const double *__restrict__ packed,
+                                                T *__restrict__ arena,
+                                                const BlockLevel *__restrict__ bls,
+                                                const Tile *__restrict__ tiles) {
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b];
+    const int i = t.x0 + threadIdx.x, j = t.y0 + threadIdx.y, k = t.z0;
+    if (i > g.n0 || j > g.n1) return;
+    arena[gidx(g, i, j, k)] =
+        (T)packed[g.poff + (i - 1) + (long long)g.n0 * ((j - 1) + (long long)g.n1 * (k - 1))];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack(const T *__restrict__ arena,
+                                              double *__restrict__ packed,
+                                              const BlockLevel *__restrict__ bls,
+                                              const Tile *__restrict__ tiles) {
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b];
+    const int i = t.x0 + threadIdx.x, j = t.y0 + threadIdx.y, k = t.z0;
+    if (i > g.n0 || j > g.n1) return;
+    packed[g.poff + (i - 1) + (long long)g.n0 * ((j - 1) + (long long)g.n1 * (k - 1))] =
+        (double)arena[gidx(g, i, j, k)];
+}
+
+}  // namespace mf3d

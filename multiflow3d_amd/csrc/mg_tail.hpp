@@ -1,0 +1,254 @@
+// mg_tail.hpp -- the coarse-level tail of the V-cycle in LDS.
+//
+// One workgroup per block runs everything from level lt down to the
+// coarsest and back up to the post-smoothing of level lt -- sweeps, ghost
+// exchange, restriction, prolongation + Neumann copy -- on level arrays held
+// in LDS, then writes level lt's potential (with ghosts) to global memory.
+// Only for blocks without neighbours or periodic partners (a lone block
+// wraps onto itself), where the exchange is local to the workgroup.  Same
+// operations, same order and same rounding as the per-level kernels.
+#pragma once
+#include "mg_common.hpp"
+
+namespace mf3d {
+
+constexpr int TAIL_MAXL = 16;
+constexpr int TAIL_NT = 512;
+
+template <typename T>
+struct TailArgs {
+    const T *cx[TAIL_MAXL];
+    const T *cy[TAIL_MAXL];
+    const T *cz[TAIL_MAXL];
+};
+
+// One level of one block inside the tail: extents, strides and the LDS
+// offsets of its potential and RHS (both in the ghosted shape).
+struct TLv {
+    int n0, n1, n2, neu, sy, sz, po, ro;
+};
+
+__device__ __forceinline__ TLv tlv_of(const BlockLevel &g, int po, int ro) {
+    TLv v;
+    v.n0 = g.n0; v.n1 = g.n1; v.n2 = g.n2; v.neu = g.neu;
+    v.sy = (int)g.sy; v.sz = (int)g.sz; v.po = po; v.ro = ro;
+    return v;
+}
+
+// Visits cells c = start, start+step, ... of an a x b x d box in x-fastest
+// order, keeping (i, j, k) by carries instead of divisions.
+struct Walk {
+    int i, j, k, si, sj, sk;
+    __device__ Walk(int start, int step, int a, int b) {
+        k = start / (a * b);
+        const int r = start - k * a * b;
+        j = r / a;
+        i = r - j * a;
+        sk = step / (a * b);
+        const int rs = step - sk * a * b;
+        sj = rs / a;
+        si = rs - sj * a;
+    }
+    __device__ void next(int a, int b) {
+        i += si;
+        if (i >= a) { i -= a; ++j; }
+        j += sj;
+        if (j >= b) { j -= b; ++k; }
+        k += sk;
+    }
+};
+
+template <typename T, bool VAR>
+__device__ __forceinline__ void tail_colour(T *sm, const TLv v, const BlockLevel &g, const T *cx,
+                                            const T *cy, const T *cz, int parity) {
+    T *P = sm + v.po;
+    const T *R = sm + v.ro;
+    const int N = v.n0 * v.n1 * v.n2;
+    Walk w(threadIdx.x, TAIL_NT, v.n0, v.n1);
+    for (int c = threadIdx.x; c < N; c += TAIL_NT, w.next(v.n0, v.n1)) {
+        const int i = w.i + 1, j = w.j + 1, k = w.k + 1;
+        if (((i + j + k) & 1) != parity) continue;
+        const int q = i + v.sy * j + v.sz * k;
+        T aw, ae, as, an, ab, at, ap;
+        coefs<T, VAR>(g, cx, cy, cz, g.off + q, i, j, k, aw, ae, as, an, ab, at, ap);
+        T s = aw * P[q - 1];
+        s = s + ae * P[q + 1];
+        s = s + as * P[q - v.sy];
+        s = s + an * P[q + v.sy];
+        s = s + ab * P[q - v.sz];
+        s = s + at * P[q + v.sz];
+        P[q] = (R[q] - s) / ap;
+    }
+}
+
+// Every ghost takes the interior cell it stands for: wrap on a periodic
+// axis, reflect on a Neumann one.
+template <typename T>
+__device__ __forceinline__ void tail_exchange(T *sm, const TLv v) {
+    T *P = sm + v.po;
+    const int a = v.n0 + 2, b = v.n1 + 2, M = a * b * (v.n2 + 2);
+    Walk w(threadIdx.x, TAIL_NT, a, b);
+    for (int c = threadIdx.x; c < M; c += TAIL_NT, w.next(a, b)) {
+        const int i = w.i, j = w.j, k = w.k;
+        const bool gi = i == 0 || i == v.n0 + 1, gj = j == 0 || j == v.n1 + 1, gk = k == 0 || k == v.n2 + 1;
+        if (!(gi || gj || gk)) continue;
+        int si = i, sj = j, sk = k;
+        if (i == 0) si = (v.neu & kNeuW) ? 1 : v.n0;
+        if (i == v.n0 + 1) si = (v.neu & kNeuE) ? v.n0 : 1;
+        if (j == 0) sj = (v.neu & kNeuS) ? 1 : v.n1;
+        if (j == v.n1 + 1) sj = (v.neu & kNeuN) ? v.n1 : 1;
+        if (k == 0) sk = (v.neu & kNeuB) ? 1 : v.n2;
+        if (k == v.n2 + 1) sk = (v.neu & kNeuT) ? v.n2 : 1;
+        P[c] = P[si + v.sy * sj + v.sz * sk];
+    }
+}
+
+// Residual of the fine level restricted into the coarse RHS (:688-751); the
+// coarse potential is zeroed including its ghosts (:650-657).
+template <typename T, bool VAR>
+__device__ __forceinline__ void tail_restrict(T *sm, const TLv vf, const BlockLevel &gf, const TLv vc,
+                                              const T *cx, const T *cy, const T *cz) {
+    const T *P = sm + vf.po;
+    const T *R = sm + vf.ro;
+    T *RC = sm + vc.ro;
+    T *PC = sm + vc.po;
+    const int MC = vc.sz * (vc.n2 + 2);
+    for (int c = threadIdx.x; c < MC; c += TAIL_NT) PC[c] = T(0);
+    const int N = vc.n0 * vc.n1 * vc.n2;
+    Walk w(threadIdx.x, TAIL_NT, vc.n0, vc.n1);
+    for (int c = threadIdx.x; c < N; c += TAIL_NT, w.next(vc.n0, vc.n1)) {
+        const int i = w.i + 1, j = w.j + 1, k = w.k + 1;
+        T acc = T(0);
+        // children in the order (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
+#pragma unroll
+        for (int ch = 0; ch < 8; ++ch) {
+            const int x = 2 * i - 1 + (ch & 1), y = 2 * j - 1 + ((ch >> 1) & 1), z = 2 * k - 1 + (ch >> 2);
+            const int q = x + vf.sy * y + vf.sz * z;
+            T aw, ae, as, an, ab, at, ap;
+            coefs<T, VAR>(gf, cx, cy, cz, gf.off + q, x, y, z, aw, ae, as, an, ab, at, ap);
+            T s = aw * P[q - 1];
+            s = s + ae * P[q + 1];
+            s = s + as * P[q - vf.sy];
+            s = s + an * P[q + vf.sy];
+            s = s + ab * P[q - vf.sz];
+            s = s + at * P[q + vf.sz];
+            s = s + ap * P[q];
+            const T r = R[q] - s;
+            acc = ch == 0 ? r : acc + r;
+        }
+        RC[i + vc.sy * j + vc.sz * k] = T(0.125) * acc;
+    }
+}
+
+// Prolongation + correction of every fine cell including ghosts, then the
+// Neumann copy (mgsolver.for:823-902).
+template <typename T>
+__device__ __forceinline__ void tail_prolong(T *sm, const TLv vf, const TLv vc) {
+    T *PF = sm + vf.po;
+    const T *PC = sm + vc.po;
+    const int a = vf.n0 + 2, b = vf.n1 + 2, M = a * b * (vf.n2 + 2);
+    {
+        Walk w(threadIdx.x, TAIL_NT, a, b);
+        for (int c = threadIdx.x; c < M; c += TAIL_NT, w.next(a, b)) {
+            const int i = w.i, j = w.j, k = w.k;
+            const int fi = i & 1, fj = j & 1, fk = k & 1;
+            const T wi0 = fi ? T(1) : T(3), wi1 = fi ? T(3) : T(1);
+            const T wj0 = fj ? T(1) : T(3), wj1 = fj ? T(3) : T(1);
+            const T wk0 = fk ? T(1) : T(3), wk1 = fk ? T(3) : T(1);
+            const int q = (i >> 1) + vc.sy * (j >> 1) + vc.sz * (k >> 1);
+            T s = (wi0 * wj0 * wk0) * PC[q];
+            s = s + (wi0 * wj1 * wk0) * PC[q + vc.sy];
+            s = s + (wi0 * wj0 * wk1) * PC[q + vc.sz];
+            s = s + (wi0 * wj1 * wk1) * PC[q + vc.sy + vc.sz];
+            s = s + (wi1 * wj0 * wk0) * PC[q + 1];
+            s = s + (wi1 * wj1 * wk0) * PC[q + 1 + vc.sy];
+            s = s + (wi1 * wj0 * wk1) * PC[q + 1 + vc.sz];
+            s = s + (wi1 * wj1 * wk1) * PC[q + 1 + vc.sy + vc.sz];
+            PF[c] = PF[c] + s / T(64);
+        }
+    }
+    __syncthreads();
+    Walk w(threadIdx.x, TAIL_NT, a, b);
+    for (int c = threadIdx.x; c < M; c += TAIL_NT, w.next(a, b)) {
+        const int i = w.i, j = w.j, k = w.k;
+        int si = i, sj = j, sk = k;
+        if (i == 0 && (vf.neu & kNeuW)) si = 1;
+        if (i == vf.n0 + 1 && (vf.neu & kNeuE)) si = vf.n0;
+        if (j == 0 && (vf.neu & kNeuS)) sj = 1;
+        if (j == vf.n1 + 1 && (vf.neu & kNeuN)) sj = vf.n1;
+        if (k == 0 && (vf.neu & kNeuB)) sk = 1;
+        if (k == vf.n2 + 1 && (vf.neu & kNeuT)) sk = vf.n2;
+        if (si != i || sj != j || sk != k) PF[c] = PF[si + vf.sy * sj + vf.sz * sk];
+    }
+}
+
+template <typename T, bool VAR>
+__device__ __forceinline__ void tail_sweeps(T *sm, const TLv v, const BlockLevel &g, const T *cx,
+                                            const T *cy, const T *cz, int n) {
+    for (int s = 0; s < n; ++s) {
+        tail_colour<T, VAR>(sm, v, g, cx, cy, cz, 1);
+        __syncthreads();
+        tail_colour<T, VAR>(sm, v, g, cx, cy, cz, 0);
+        __syncthreads();
+        tail_exchange<T>(sm, v);
+        __syncthreads();
+    }
+}
+
+template <typename T, bool VAR>
+__global__ __launch_bounds__(TAIL_NT) void k_tail(const BlockLevel *__restrict__ tl, int nb, int nlev,
+                                                  const T *__restrict__ rhs_in,
+                                                  const T *__restrict__ phi_in, T *__restrict__ phi_out,
+                                                  TailArgs<T> co, int nu_pre, int nu_post,
+                                                  const int *__restrict__ done) {
+    if (done && *done) return;
+    extern __shared__ unsigned char smem_raw[];
+    T *sm = reinterpret_cast<T *>(smem_raw);
+    __shared__ TLv lv[TAIL_MAXL];
+    const int b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int li = 0; li < nlev; ++li) {
+            const BlockLevel &g = tl[li * nb + b];
+            lv[li] = tlv_of(g, o, 0);
+            o += (int)(g.sz * (g.n2 + 2));
+        }
+        for (int li = 0; li < nlev; ++li) {
+            lv[li].ro = o;
+            o += lv[li].sz * (lv[li].n2 + 2);
+        }
+    }
+    __syncthreads();
+    {
+        const TLv v = lv[0];
+        const long long goff = tl[b].off;
+        const int M = v.sz * (v.n2 + 2);
+        for (int c = threadIdx.x; c < M; c += TAIL_NT) {
+            sm[v.ro + c] = rhs_in[goff + c];
+            sm[v.po + c] = phi_in ? phi_in[goff + c] : T(0);
+        }
+    }
+    __syncthreads();
+    for (int li = 0; li < nlev; ++li) {
+        if (li > 0) {
+            tail_restrict<T, VAR>(sm, lv[li - 1], tl[(li - 1) * nb + b], lv[li], co.cx[li - 1], co.cy[li - 1],
+                                  co.cz[li - 1]);
+            __syncthreads();
+        }
+        tail_sweeps<T, VAR>(sm, lv[li], tl[li * nb + b], co.cx[li], co.cy[li], co.cz[li],
+                            nu_pre + (li == nlev - 1 ? nu_post : 0));
+    }
+    for (int li = nlev - 2; li >= 0; --li) {
+        tail_prolong<T>(sm, lv[li], lv[li + 1]);
+        __syncthreads();
+        tail_sweeps<T, VAR>(sm, lv[li], tl[li * nb + b], co.cx[li], co.cy[li], co.cz[li], nu_post);
+    }
+    {
+        const TLv v = lv[0];
+        const long long goff = tl[b].off;
+        const int M = v.sz * (v.n2 + 2);
+        for (int c = threadIdx.x; c < M; c += TAIL_NT) phi_out[goff + c] = sm[v.po + c];
+    }
+}
+
+}  // namespace mf3d

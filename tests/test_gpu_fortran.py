@@ -1,0 +1,61 @@
+"""The Fortran boundary end to end: a Fortran host (multiflow3d_amd/fortran)
+holding MultiFlow3D-style arrays with two ghost layers calls the library
+through the ISO_C_BINDING module; its pressure must equal the CPU oracle's
+bitwise."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from multiflow3d_amd.build import FORTRAN_HOST
+from oracle.oracle import OracleMG
+
+from .cases import case, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_host(tmp_path, grid, L, f, dens, maxcy, tol, lsm):
+    rhs = tmp_path / "rhs.bin"
+    np.concatenate([a.ravel() for a in f]).tofile(rhs)
+    dfile = "-"
+    if lsm:
+        dfile = str(tmp_path / "dens.bin")
+        np.concatenate([a.ravel() for a in dens]).tofile(dfile)
+    out = tmp_path / "out.bin"
+    ctl = tmp_path / "ctl.txt"
+    nbx, nby, nbz = grid.shape
+    ctl.write_text(f"{nbx} {nby} {nbz} {L} {int(lsm)} {maxcy} 1 2 2\n{tol!r}\n"
+                   + " ".join(map(str, grid.bc)) + "\n" + " ".join(map(str, grid.ex)) + "\n"
+                   + " ".join(map(str, grid.ey)) + "\n" + " ".join(map(str, grid.ez)) + "\n"
+                   + " ".join(repr(float(x)) for x in grid.h) + f"\n{rhs}\n{dfile}\n{out}\n")
+    res = subprocess.run([FORTRAN_HOST, str(ctl)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    k = int(raw[0])
+    hist = raw[1:k + 2]
+    p, o = [], k + 3
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        m = nx * ny * nz
+        p.append(raw[o:o + m].reshape(nz, ny, nx))
+        o += m
+    return p, hist, k
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_HOST), reason="Fortran host not built")
+@pytest.mark.parametrize("name", ["single16_L4", "multi_uneven_L3", "lsm_multi_L3"])
+def test_fortran_host_matches_oracle(tmp_path, name):
+    grid, L, lsm = case(name)
+    f, dens = inputs(grid, lsm)
+    o = OracleMG(grid.blocks, L, lsm=lsm)
+    if lsm:
+        o.set_density(dens)
+    o.coeff()
+    p_o, rho_o, _, k_o = o.solve(f, 0.0, 4)
+    p_f, hist, k = _run_host(tmp_path, grid, L, f, dens, 4, 0.0, lsm)
+    assert k == k_o == 4
+    np.testing.assert_allclose(hist, rho_o, rtol=1e-12, atol=0)
+    for a, b in zip(p_f, p_o):
+        assert np.array_equal(a, b)
