@@ -46,11 +46,13 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     f_dev = torch.from_numpy(np.ascontiguousarray(np.concatenate([a.ravel() for a in f_host]))).to(dev)
     p_dev = torch.zeros_like(f_dev)
     mg = PressureMultigrid(grid, levels, maxcy=maxcy, eps=tol, precision=precision)
+    if time_kernel:
+        mg.time_kernel(_lib.MF3D_K_SMOOTH)
     for _ in range(warmup):
         mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr())
     torch.cuda.synchronize()
     if time_kernel:
-        mg.time_kernel(_lib.MF3D_K_SMOOTH)
+        mg.time_kernel(_lib.MF3D_K_SMOOTH)  # resets the counters
     lat = []
     res = None
     torch.cuda.synchronize()
@@ -62,10 +64,12 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     launches, kms = mg.kernel_time() if time_kernel else (0, 0.0)
+    replay = mg.graph_replay()
     mg.time_kernel(0)
     p = p_dev.cpu().numpy()
     out = dict(ms=(t1 - t0) * 1e3 / steps, lat=lat, iters=res.iters, rho=float(res.hist[-1]),
-               status=res.status, p=p, launches=launches, kernel_ms=kms)
+               status=res.status, p=p, launches=launches, kernel_ms=kms, graph_replay=replay,
+               hook_replay=replay if time_kernel else None)
     mg.close()
     return out
 
@@ -80,11 +84,14 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxcy", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the solves, copy MIB MiB device-to-device once (a copy of known size "
+                         "to scale HBM byte counters read by a profiler)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(independent solves shard by request; see DESIGN.md)", file=sys.stderr)
+              "(independent solves shard across GPUs by request)", file=sys.stderr)
         sys.exit(2)
 
     from multiflow3d_amd import topology as T
@@ -97,8 +104,16 @@ def main():
     levels_cells = [(n >> l) ** 3 for l in range(L)]
 
     r64 = run_gpu(grid, L, f_host, "fp64", args.steps, args.warmup, args.tol, args.maxcy, True)
+    if not r64["graph_replay"]:
+        # the runtime refused to capture the timing events: keep the hook's
+        # kernel times, take the solve rate from graph replay without it
+        t = run_gpu(grid, L, f_host, "fp64", args.steps, args.warmup, args.tol, args.maxcy, False)
+        r64.update(ms=t["ms"], lat=t["lat"], graph_replay=t["graph_replay"])
     print(f"# fp64: {r64['ms']:.3f} ms/solve, {r64['iters']} cycles, rho={r64['rho']:.3e}", file=sys.stderr, flush=True)
     rmx = run_gpu(grid, L, f_host, "mixed", args.steps, args.warmup, args.tol, args.maxcy, True)
+    if not rmx["graph_replay"]:
+        t = run_gpu(grid, L, f_host, "mixed", args.steps, args.warmup, args.tol, args.maxcy, False)
+        rmx.update(ms=t["ms"], lat=t["lat"], graph_replay=t["graph_replay"])
     print(f"# mixed: {rmx['ms']:.3f} ms/solve, {rmx['iters']} cycles, rho={rmx['rho']:.3e}", file=sys.stderr, flush=True)
 
     def rel_l2_mean_removed(a, b):
@@ -108,20 +123,40 @@ def main():
 
     mixed_err = rel_l2_mean_removed(rmx["p"], r64["p"])
 
-    # dominant kernel: one colour of the red-black sweep on the finest level
-    # algorithmic bytes per launch = N/2 reads of the other colour, N/2 RHS
-    # reads and N/2 writes of this colour, w bytes each
+    if args.calib_copy > 0:
+        import torch
+        nbytes = args.calib_copy << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {nbytes} bytes read, {nbytes} bytes written", file=sys.stderr, flush=True)
+        del a, b
+
+    # Dominant kernel: k_rbs, one fused red-black sweep (both colours) of the
+    # finest level.  Algorithmic bytes: a sweep reads phi and the RHS and
+    # writes phi once per interior cell, 3 w N; the first sweep of each
+    # cycle starts from phi = 0 and reads only the RHS, 2 w N.  V(2,2) runs
+    # 4 finest sweeps per cycle, one of them from zero: 2.75 w N per launch
+    # on average over the launches the hook times.
     N0 = levels_cells[0]
 
     def roofline(r, w):
         if not r["launches"]:
             return None
         avg_s = r["kernel_ms"] / r["launches"] * 1e-3
-        achieved = 1.5 * w * N0 / avg_s / 1e9
+        per_launch = 2.75 * w * N0
+        achieved = per_launch / avg_s / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_rb (one colour, finest level)", "avg_us": round(avg_s * 1e6, 2),
-                "bytes_per_launch": int(1.5 * w * N0)}
+                "kernel": "k_rbs (fused red-black sweep, finest level)", "avg_us": round(avg_s * 1e6, 2),
+                "launches_timed": r["launches"], "bytes_per_launch": int(per_launch),
+                "timed_with": "HIP events on the solver's stream around each launch, inside the timed "
+                              "region (graph replay)" if r["hook_replay"] else
+                              "HIP events on the solver's stream around each launch, over the same "
+                              "solves launched eagerly, the step time taken from a second pass with "
+                              "graph replay (events recorded by a graph cannot be timed here)"}
 
     b64 = algorithmic_bytes_per_cycle(levels_cells, 8) * r64["iters"]
     bmx = algorithmic_bytes_per_cycle(levels_cells, 4) * rmx["iters"]
@@ -165,6 +200,7 @@ def main():
                  "cycles": r64["iters"], "final_rho": r64["rho"],
                  "alg_GBps": round(b64 / (r64["ms"] * 1e-3) / 1e9, 1),
                  "alg_frac_of_peak": round(b64 / (r64["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                 "graph_replay": r64["graph_replay"],
                  "roofline": roofline(r64, 8)},
         "mixed": {"solves_per_s": round(1e3 / rmx["ms"], 4), "ms_per_solve": round(rmx["ms"], 4),
                   "cycles": rmx["iters"], "final_rho": rmx["rho"],
@@ -172,7 +208,7 @@ def main():
                   "alg_frac_of_peak": round(bmx / (rmx["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
                   "p50_ms": round(float(np.percentile(lat, 50)), 3),
                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                  "rel_l2_vs_fp64": mixed_err},
+                  "rel_l2_vs_fp64": mixed_err, "graph_replay": rmx["graph_replay"]},
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
     }
     print(json.dumps(line), flush=True)

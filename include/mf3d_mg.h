@@ -116,12 +116,21 @@ int64_t mf3d_mg_estimate_bytes(const mf3d_mg_cfg *cfg, const mf3d_block_desc *bl
 
 /* Measurement hook (no counterpart in the reference; for benchmarks):
  * while enabled, every finest-level launch of the chosen kernel is
- * bracketed by HIP events on the stream it runs on.  mf3d_mg_time_kernel
- * selects the kernel (MF3D_K_*; 0 = off) and resets the counters;
- * mf3d_mg_kernel_time returns the launches seen and their summed time. */
+ * bracketed by HIP events on the stream it runs on.  Each cycle index is
+ * then replayed from a captured graph of its own that records its own
+ * events, so timing keeps graph replay (where the runtime refuses to
+ * capture event records, the cycles are launched eagerly instead; see
+ * mf3d_mg_graph_replay).  Cycles skipped after convergence are not
+ * counted.  mf3d_mg_time_kernel selects the kernel (MF3D_K_*; 0 = off)
+ * and resets the counters; mf3d_mg_kernel_time returns the launches seen
+ * and their summed time. */
 enum { MF3D_K_OFF = 0, MF3D_K_SMOOTH = 1, MF3D_K_OUTER = 2, MF3D_K_RESTRICT = 3, MF3D_K_PROLONG = 4 };
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel);
 int mf3d_mg_kernel_time(mf3d_mg *mg, int64_t *launches, double *total_ms);
+
+/* 1 if the last solve replayed captured cycle graphs, 0 if it launched its
+ * kernels one by one (no counterpart in the reference; for benchmarks). */
+int mf3d_mg_graph_replay(const mf3d_mg *mg);
 
 const char *mf3d_mg_last_error(const mf3d_mg *mg);
 void mf3d_mg_destroy(mf3d_mg *mg);

@@ -21,6 +21,7 @@ EXPORTS = (
     "mf3d_mg_solve_device", "mf3d_mg_cells", "mf3d_mg_device_bytes",
     "mf3d_mg_estimate_bytes", "mf3d_mg_last_error", "mf3d_mg_destroy",
     "mf3d_mg_abi_version", "mf3d_mg_time_kernel", "mf3d_mg_kernel_time",
+    "mf3d_mg_graph_replay",
 )
 MF3D_K_OFF, MF3D_K_SMOOTH, MF3D_K_OUTER, MF3D_K_RESTRICT, MF3D_K_PROLONG = 0, 1, 2, 3, 4
 
@@ -85,5 +86,7 @@ def lib():
     L.mf3d_mg_time_kernel.argtypes = [vp, ctypes.c_int]
     L.mf3d_mg_kernel_time.restype = ctypes.c_int
     L.mf3d_mg_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
+    L.mf3d_mg_graph_replay.restype = ctypes.c_int
+    L.mf3d_mg_graph_replay.argtypes = [vp]
     _lib = L
     return L

@@ -90,10 +90,22 @@ struct mf3d_mg {
     int n_out = 0;
     bool graphs_ready = false;
     int guess = 0;  // cycles the previous solve needed
-    // kernel timing hook
+    // kernel timing hook: event pairs around each finest-level launch of
+    // the selected kernel.  Eager launches record into ev_eager; cycle k is
+    // replayed from its own captured graph tkg[k], which records into its
+    // own events tkev[k], so timing keeps graph replay.
+    struct EvList {
+        std::vector<hipEvent_t> ev;
+        size_t used = 0;
+    };
     int tk = 0;
-    std::vector<hipEvent_t> ev;
-    size_t ev_used = 0;
+    EvList ev_eager;
+    EvList *ev_rec = nullptr;  // where tk_mark records (nullptr: ev_eager)
+    bool ev_fail = false;      // an event record failed (e.g. refused during capture)
+    std::vector<hipGraphExec_t> tkg;
+    std::vector<EvList> tkev;
+    bool tk_eager = false;  // timed graphs unavailable: time eager launches
+    int last_replay = 0;    // the last solve replayed captured cycles
     long long tk_launches = 0;
     double tk_ms = 0.0;
 };
@@ -292,24 +304,47 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
 inline bool tk_on(const mf3d_mg *mg, int kernel, int level) { return mg->tk == kernel && level == 0; }
 
 void tk_mark(mf3d_mg *mg) {
-    if (mg->ev_used == mg->ev.size()) {
+    mf3d_mg::EvList &E = mg->ev_rec ? *mg->ev_rec : mg->ev_eager;
+    if (E.used == E.ev.size()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        mg->ev.push_back(e);
+        if (hipEventCreate(&e) != hipSuccess) {
+            mg->ev_fail = true;
+            return;
+        }
+        E.ev.push_back(e);
     }
-    (void)hipEventRecord(mg->ev[mg->ev_used++], mg->stream);
+    if (hipEventRecord(E.ev[E.used++], mg->stream) != hipSuccess) mg->ev_fail = true;
 }
 
-// Call after the stream is synchronised: fold the recorded pairs into the totals.
-void tk_flush(mf3d_mg *mg) {
-    for (size_t i = 0; i + 1 < mg->ev_used; i += 2) {
+// Call after the stream is synchronised: fold the recorded pairs into the
+// totals.  A captured list keeps its events (its graph records them again).
+void tk_flush(mf3d_mg *mg, mf3d_mg::EvList &E, bool captured) {
+    for (size_t i = 0; i + 1 < E.used; i += 2) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, mg->ev[i], mg->ev[i + 1]) == hipSuccess) {
+        const hipError_t e = hipEventElapsedTime(&ms, E.ev[i], E.ev[i + 1]);
+        if (e == hipSuccess) {
             mg->tk_ms += ms;
             mg->tk_launches += 1;
+        } else {
+            (void)hipGetLastError();  // not a device error: the pair is just not counted
+            if (captured && !mg->tk_eager) {
+                std::fprintf(stderr, "mf3d_mg: no timing from events recorded by a graph (%s); "
+                                     "timing eager launches from the next solve\n", hipGetErrorString(e));
+                mg->tk_eager = true;  // the timed graphs are dropped at the next solve
+            }
         }
     }
-    mg->ev_used = 0;
+    if (!captured) E.used = 0;
+}
+void tk_flush(mf3d_mg *mg) { tk_flush(mg, mg->ev_eager, false); }
+
+void drop_timed_graphs(mf3d_mg *mg) {
+    for (hipGraphExec_t &g : mg->tkg)
+        if (g) (void)hipGraphExecDestroy(g);
+    mg->tkg.clear();
+    for (mf3d_mg::EvList &E : mg->tkev)
+        for (hipEvent_t e : E.ev) (void)hipEventDestroy(e);
+    mg->tkev.clear();
 }
 
 // ---------------------------------------------------------------- launches
@@ -331,16 +366,19 @@ void gather(mf3d_mg *mg, int l, const int *done) {
                        L.r_full, done);
 }
 
+// The mixed form's FP32 inner cycle multiplies by 1/ap; the FP64 form
+// divides, as the reference does.
 template <typename T, bool VAR, bool ZERO>
 void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
+    constexpr bool RCP = sizeof(T) == 4;
     const T *pin = (const T *)L.phi[L.cur];
     T *pout = (T *)L.phi[1 - L.cur];
     if (L.sw_ty == 8)
-        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db, L.t_sw,
                            done);
     else
-        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db, L.t_sw,
                            done);
 }
@@ -468,15 +506,68 @@ int capture_cycles(mf3d_mg *mg) {
     return 0;
 }
 
+// Record cycle k (reading pbuf[k % 2]) with the timing hook's events into
+// its own graph.  Returns 1 when the runtime refuses to capture the event
+// records; the hook then times eager launches instead.
+template <typename T, bool VAR>
+int capture_timed(mf3d_mg *mg, int k) {
+    if ((int)mg->tkg.size() <= k) {
+        mg->tkg.resize(k + 1, nullptr);
+        mg->tkev.resize(k + 1);
+    }
+    if (mg->tkg[k]) return 0;
+    for (Level &V : mg->lv) V.cur = 0;
+    mg->tkev[k].used = 0;
+    // events are created before the capture, none while it runs
+    for (size_t want = 2 * (size_t)(mg->cfg.nu_pre + mg->cfg.nu_post + 2); mg->tkev[k].ev.size() < want;) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        mg->tkev[k].ev.push_back(e);
+    }
+    mg->ev_rec = &mg->tkev[k];
+    mg->ev_fail = false;
+    HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
+    vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
+    outer_step<T, VAR>(mg, k % 2, done_ptr(mg));
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(mg->stream, &graph);
+    mg->ev_rec = nullptr;
+    if (e != hipSuccess || mg->ev_fail || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return 1;
+    }
+    e = hipGraphInstantiate(&mg->tkg[k], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        mg->tkg[k] = nullptr;
+        (void)hipGetLastError();
+        return 1;
+    }
+    return 0;
+}
+
 template <typename T, bool VAR>
 int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rmax_out, int32_t *iters_out,
               double *hist) {
     Level &L0 = mg->lv[0];
+    const int maxcy = mg->cfg.maxcy;
+    int chunk = mg->guess > 0 ? mg->guess : 4;
     const bool use_graph = mg->tk == MF3D_K_OFF;
     if (use_graph && !mg->graphs_ready) {
         int rc = capture_cycles<T, VAR>(mg);
         if (rc) return rc;
     }
+    // timed graphs for the first chunk, captured while the stream is idle
+    if (mg->tk_eager) drop_timed_graphs(mg);
+    bool timed = !use_graph && !mg->tk_eager;
+    for (int k = 1; timed && k <= std::min(chunk, maxcy); ++k)
+        if (capture_timed<T, VAR>(mg, k)) {
+            drop_timed_graphs(mg);
+            mg->tk_eager = true;
+            timed = false;
+        }
+    mg->last_replay = use_graph || timed;
     hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st);
     fill<double>(mg, mg->pbuf[0], L0.cells_gh);
     fill<T>(mg, (T *)L0.phi[L0.cur], L0.cells_gh);
@@ -484,17 +575,23 @@ int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rma
                        L0.db, L0.t_in);
     // step 0: r_0 = f - A*0 from pbuf[0] into pbuf[1]
     outer_step<T, VAR>(mg, 0, nullptr);
-    const int maxcy = mg->cfg.maxcy;
     int launched = 0;
     SolveState hs{};
     hs.done = 0;
-    int chunk = mg->guess > 0 ? mg->guess : 4;
     while (launched < maxcy) {
         const int n = std::min(chunk, maxcy - launched);
+        // later chunks follow a synchronisation: the stream is idle here
+        for (int c = 0; timed && c < n; ++c)
+            if (capture_timed<T, VAR>(mg, launched + c + 1)) {
+                mg->err = "capture of the timing hook's events failed after earlier captures succeeded";
+                return MF3D_EDEVICE;
+            }
         for (int c = 0; c < n; ++c) {
             const int k = launched + c + 1;  // cycle k reads pbuf[k % 2]
             if (use_graph) {
                 HIPCHK(hipGraphLaunch(mg->cyc[k % 2], mg->stream));
+            } else if (timed) {
+                HIPCHK(hipGraphLaunch(mg->tkg[k], mg->stream));
             } else {
                 for (Level &V : mg->lv) V.cur = 0;
                 vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
@@ -505,6 +602,9 @@ int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rma
         HIPCHK(hipMemcpyAsync(&hs, mg->st, sizeof(SolveState), hipMemcpyDeviceToHost, mg->stream));
         HIPCHK(hipStreamSynchronize(mg->stream));
         tk_flush(mg);
+        // cycles after convergence skip their work: their events are not counted
+        const int ran = hs.done ? hs.last : launched;
+        for (int k = launched - n + 1; timed && k <= ran; ++k) tk_flush(mg, mg->tkev[k], true);
         if (hs.done) break;
         chunk = 1;
     }
@@ -673,7 +773,9 @@ int build(mf3d_mg *mg) {
             g.gz = B.h[0] * B.h[1] / B.h[2];
         }
         V.cells_gh = off;
-        V.sw_ty = (n1max >= 8) ? 8 : 4;
+        // the finest level marches 8-row tiles; coarser levels have few
+        // columns and use 4-row tiles, which keeps their z chunks deeper
+        V.sw_ty = (l == 0 && n1max >= 8) ? 8 : 4;
         if ((rc = upload(mg, &V.db, V.hb))) return rc;
         auto tsw = sweep_tiles(V.hb, V.sw_ty), tin = cell_tiles(V.hb), trow = row_tiles(V.hb);
         V.n_sw = (int)tsw.size(); V.n_in = (int)tin.size(); V.n_row = (int)trow.size();
@@ -797,6 +899,7 @@ void drop_graphs(mf3d_mg *mg) {
             g = nullptr;
         }
     mg->graphs_ready = false;
+    drop_timed_graphs(mg);
 }
 
 }  // namespace
@@ -925,6 +1028,7 @@ int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel) {
     if (!mg || kernel < MF3D_K_OFF || kernel > MF3D_K_PROLONG) return MF3D_EINVAL;
     if (mg->stream) (void)hipStreamSynchronize(mg->stream);
     tk_flush(mg);
+    if (kernel != mg->tk) drop_timed_graphs(mg);
     mg->tk = kernel;
     mg->tk_launches = 0;
     mg->tk_ms = 0.0;
@@ -940,6 +1044,8 @@ int mf3d_mg_kernel_time(mf3d_mg *mg, int64_t *launches, double *total_ms) {
     return 0;
 }
 
+int mf3d_mg_graph_replay(const mf3d_mg *mg) { return mg ? mg->last_replay : MF3D_EINVAL; }
+
 int64_t mf3d_mg_cells(const mf3d_mg *mg) { return mg ? mg->cells : MF3D_EINVAL; }
 int64_t mf3d_mg_device_bytes(const mf3d_mg *mg) { return mg ? mg->bytes : MF3D_EINVAL; }
 
@@ -954,7 +1060,7 @@ void mf3d_mg_destroy(mf3d_mg *mg) {
     drop_graphs(mg);
     for (void *p : mg->allocs) (void)hipFree(p);
     if (mg->h_io) (void)hipHostFree(mg->h_io);
-    for (hipEvent_t e : mg->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : mg->ev_eager.ev) (void)hipEventDestroy(e);
     if (mg->stream) (void)hipStreamDestroy(mg->stream);
     delete mg;
 }

@@ -16,18 +16,25 @@ namespace mf3d {
 // planes z0..z1.  LDS keeps a ring of 5 planes of pin, each with a 2-cell
 // halo in x and y.  At step c the first colour is computed on plane c over
 // the tile plus a 1-cell ring (in place in LDS), then the second colour on
-// plane c-1 over the tile, which is written out.  Red cells in the ring are
+// plane c-1: half the threads compute its cells, the other half copy the
+// first-colour cells of that plane out of LDS.  Red cells in the ring are
 // recomputed by both neighbouring workgroups from the same inputs, so they
 // are bitwise the values the reference computes once.
 //
-// Loads run one step ahead: while step c computes, the registers already
-// hold plane c+2 of pin and the RHS values of the next red and black cells,
-// so global-memory latency overlaps the LDS phases instead of stalling them.
+// Loads run ahead of use: two register sets alternate between even and odd
+// steps, so while step c computes, plane c+2 of pin and the RHS of the next
+// cells are in flight; loads are unconditional (clamped address, masked
+// result) so the compiler waits only for the one it needs.  Addresses and
+// LDS indices are precomputed per thread for both plane parities.  Where no
+// Neumann-zeroed coefficient is in reach of the tile (the common case) the
+// constant coefficients and ap are hoisted.
 // ZERO: pin is identically zero (the first sweep after restriction, where
 // the reference has just zeroed phi, mgsolver.for:650-657).
+// RCP: multiply by 1/ap instead of dividing (inner cycle of the mixed
+// precision form only; the FP64 form divides, as the reference does).
 constexpr int SW_TX = 64;
 
-template <typename T, bool VAR, bool ZERO, int TY>
+template <typename T, bool VAR, bool ZERO, bool RCP, int TY>
 __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T *__restrict__ pout,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ cx,
@@ -39,8 +46,8 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
     if (done && *done) return;
     constexpr int LX = SW_TX + 4, LY = TY + 4, NS = 5, NT = SW_TX * TY, LXY = LX * LY;
     constexpr int KL = (LXY + NT - 1) / NT;
-    constexpr int HALF = (SW_TX + 2) / 2;
-    constexpr int PD = 3;  // load pipeline depth in plane steps
+    constexpr int HALF = (SW_TX + 2) / 2;  // first-colour cells per ring row
+    constexpr int NB = NT / 2;             // second-colour cells per tile plane
     __shared__ T sp[NS][LXY];
     const Tile t = tiles[blockIdx.x];
     const BlockLevel g = bls[t.b];
@@ -48,8 +55,9 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
     const int xb = t.x0 - 2, yb = t.y0 - 2;
     const int zlo = t.z0 - 2 < 0 ? 0 : t.z0 - 2;
     const int zhi = t.z1 + 2 > g.n2 + 1 ? g.n2 + 1 : t.z1 + 2;
+    const long long sz = g.sz;
 
-    // this thread's share of a plane load
+    // this thread's share of a plane load (clamped address when outside)
     long long lofs[KL];
     bool lok[KL];
 #pragma unroll
@@ -58,53 +66,108 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
         const int ly = p / LX, lx = p - ly * LX;
         const int x = xb + lx, y = yb + ly;
         lok[k] = p < LXY && x >= 0 && x <= g.n0 + 1 && y >= 0 && y <= g.n1 + 1;
-        lofs[k] = g.off + (long long)x + g.sy * y;
+        lofs[k] = lok[k] ? g.off + (long long)x + g.sy * y : g.off + g.sy + 1;
     }
-    // Loads are issued unconditionally from a clamped address and masked
-    // afterwards: loads under divergent branches make the compiler wait for
-    // all outstanding loads instead of just the one it needs.
-#pragma unroll
-    for (int k = 0; k < KL; ++k)
-        if (!lok[k]) lofs[k] = g.off + g.sy + 1;
 
-    // this thread's red cell (the ring position depends on the plane's parity)
-    const bool has_red = tid < HALF * (TY + 2);
+    // first-colour (red) ring cell of this thread, per plane parity
     const int r_row = tid / HALF, r_m = tid - r_row * HALF;
     const int ry = t.y0 - 1 + r_row, rxs = t.x0 - 1;
-    auto red_x = [&](int c) { return rxs + ((1 - ry - c - rxs) & 1) + 2 * r_m; };
-    auto red_ok = [&](int c) {
-        const int x = red_x(c);
-        return has_red && c >= 1 && c <= g.n2 && x >= 1 && x <= g.n0 && ry >= 1 && ry <= g.n1;
-    };
-    // this thread's black-phase column
-    const int bx = t.x0 + threadIdx.x, by = t.y0 + threadIdx.y;
-    const bool has_col = bx <= g.n0 && by <= g.n1;
+    bool rok[2];
+    int rl[2];
+    long long rg[2];
+    int rbnd[2];  // x/y boundary position bits for the coefficients
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        const int x = rxs + ((1 - ry - par - rxs) & 1) + 2 * r_m;
+        rok[par] = tid < HALF * (TY + 2) && x >= 1 && x <= g.n0 && ry >= 1 && ry <= g.n1;
+        rl[par] = (x - xb) + LX * (ry - yb);
+        rg[par] = rok[par] ? g.off + (long long)x + g.sy * ry : g.off;
+        rbnd[par] = x | (ry << 16);
+    }
+    // second phase: threads < NB compute second-colour cells, the others
+    // copy first-colour cells of the same plane out of LDS
+    const bool blk = tid < NB;
+    const int bt = blk ? tid : tid - NB;
+    const int b_row = bt / (SW_TX / 2), b_m = bt - b_row * (SW_TX / 2);
+    const int by = t.y0 + b_row;
+    bool bok[2];
+    int bl[2];
+    long long bg[2];
+    int bbnd[2];
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        const int e = (1 + by + par) & 1;  // offset of the second-colour cell in its pair
+        const int x = t.x0 + 2 * b_m + (blk ? e : 1 - e);
+        bok[par] = x <= g.n0 && by <= g.n1;
+        bl[par] = (x - xb) + LX * (by - yb);
+        bg[par] = bok[par] ? g.off + (long long)x + g.sy * by : g.off;
+        bbnd[par] = x | (by << 16);
+    }
 
-    // Register sets for a PD-deep load pipeline: set s serves the steps
-    // c == c0 + s (mod PD).  At step c it supplies plane c+1 of pin (stored
-    // to LDS), the RHS of this thread's red cell on plane c and of its black
-    // cell on plane c-1, and is then reloaded with the values step c+PD
-    // needs -- consumed, then overwritten, never copied.
+    // hoisted constant coefficients where no Neumann zero is in reach
+    const bool xy_clean = !VAR && (!(g.neu & kNeuW) || t.x0 > 2) && (!(g.neu & kNeuE) || t.x0 + SW_TX < g.n0) &&
+                          (!(g.neu & kNeuS) || t.y0 > 2) && (!(g.neu & kNeuN) || t.y0 + TY < g.n1);
+    const T ca = cconst<T>(g, 0), cb = cconst<T>(g, 1), cc = cconst<T>(g, 2);
+    T ap0;
+    {
+        T s0 = ca + ca;
+        s0 = s0 + cb;
+        s0 = s0 + cb;
+        s0 = s0 + cc;
+        s0 = s0 + cc;
+        ap0 = -s0;
+    }
+    const T rap0 = T(1) / ap0;
+    auto z_clean = [&](int z) {
+        return (!(g.neu & kNeuB) || z > 1) && (!(g.neu & kNeuT) || z < g.n2);
+    };
+    // one Gauss-Seidel update from the LDS planes P (this), Pm, Pp
+    auto update = [&](const T *P, const T *Pm, const T *Pp, int l, int bnd, int z, long long q, T r,
+                      bool fast) -> T {
+        if (fast) {
+            T s = ca * P[l - 1];
+            s = s + ca * P[l + 1];
+            s = s + cb * P[l - LX];
+            s = s + cb * P[l + LX];
+            s = s + cc * Pm[l];
+            s = s + cc * Pp[l];
+            return RCP ? (r - s) * rap0 : (r - s) / ap0;
+        }
+        T aw, ae, as, an, ab, at, ap;
+        coefs<T, VAR>(g, cx, cy, cz, q, bnd & 0xffff, bnd >> 16, z, aw, ae, as, an, ab, at, ap);
+        T s = aw * P[l - 1];
+        s = s + ae * P[l + 1];
+        s = s + as * P[l - LX];
+        s = s + an * P[l + LX];
+        s = s + ab * Pm[l];
+        s = s + at * Pp[l];
+        return RCP ? (r - s) * (T(1) / ap) : (r - s) / ap;
+    };
+
     struct Regs {
         T pre[KL];
         T rr, rb;
     };
-    auto load_regs = [&](Regs &R, int c) {  // the values step c needs
-        const int zp = c + 1 < zhi ? c + 1 : zhi;
+    // the values step c needs: plane c+1 of pin, the RHS of this thread's red
+    // cell on plane c and of its second-phase cell on plane c-1
+    auto load_regs = [&](Regs &R, int c) {
+        const long long zo = sz * (long long)(c + 1 < zhi ? c + 1 : zhi);
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             if (ZERO) {
                 R.pre[k] = T(0);
             } else {
-                const T v = pin[lofs[k] + g.sz * zp];
+                const T v = pin[lofs[k] + zo];
                 R.pre[k] = lok[k] ? v : T(0);
             }
         }
-        const bool ok = red_ok(c);
-        const T v = rhs[ok ? gidx(g, red_x(c), ry, c) : g.off];
-        R.rr = ok ? v : T(0);
-        const bool okb = has_col && c - 1 >= t.z0 && c - 1 <= t.z1;
-        const T w = rhs[okb ? gidx(g, bx, by, c - 1) : g.off];
+        const int pr = c & 1;
+        const bool okr = rok[pr] && c >= 1 && c <= g.n2;
+        const T v = rhs[(okr ? rg[pr] : g.off) + (okr ? sz * c : 0)];
+        R.rr = okr ? v : T(0);
+        const int pb = (c - 1) & 1;
+        const bool okb = blk && bok[pb] && c - 1 >= t.z0 && c - 1 <= t.z1;
+        const T w = rhs[(okb ? bg[pb] : g.off) + (okb ? sz * (c - 1) : 0)];
         R.rb = okb ? w : T(0);
     };
 
@@ -115,7 +178,7 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
 #pragma unroll
             for (int k = 0; k < KL; ++k) {
                 if (tid + k * NT >= LXY) continue;
-                P[tid + k * NT] = (!ZERO && lok[k]) ? pin[lofs[k] + g.sz * c] : T(0);
+                P[tid + k * NT] = (!ZERO && lok[k]) ? pin[lofs[k] + sz * c] : T(0);
             }
         }
 
@@ -127,63 +190,43 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
                 if (tid + k * NT < LXY) P[tid + k * NT] = R.pre[k];
         }
         const T rr_cur = R.rr, rb_cur = R.rb;
-        load_regs(R, c + PD);
+        load_regs(R, c + 2);
         __syncthreads();
-        if (red_ok(c)) {
-            T *P = sp[c % NS];
-            const T *Pm = sp[(c + NS - 1) % NS];
-            const T *Pp = sp[(c + 1) % NS];
-            const int x = red_x(c);
-            const int l = (x - xb) + LX * (ry - yb);
-            T aw, ae, as, an, ab, at, ap;
-            coefs<T, VAR>(g, cx, cy, cz, gidx(g, x, ry, c), x, ry, c, aw, ae, as, an, ab, at, ap);
-            T s = aw * P[l - 1];
-            s = s + ae * P[l + 1];
-            s = s + as * P[l - LX];
-            s = s + an * P[l + LX];
-            s = s + ab * Pm[l];
-            s = s + at * Pp[l];
-            P[l] = (rr_cur - s) / ap;
+        {
+            const int pr = c & 1;
+            if (rok[pr] && c >= 1 && c <= g.n2) {
+                T *P = sp[c % NS];
+                const int l = rl[pr];
+                P[l] = update(P, sp[(c + NS - 1) % NS], sp[(c + 1) % NS], l, rbnd[pr], c,
+                              rg[pr] + sz * c, rr_cur, xy_clean && z_clean(c));
+            }
         }
         __syncthreads();
-        const int z = c - 1;
-        if (has_col && z >= t.z0 && z <= t.z1) {
+        const int z = c - 1, pz = z & 1;
+        if (bok[pz] && z >= t.z0 && z <= t.z1) {
             const T *P = sp[z % NS];
-            const int l = (bx - xb) + LX * (by - yb);
-            const long long q = gidx(g, bx, by, z);
+            const int l = bl[pz];
+            const long long q = bg[pz] + sz * z;
             T v;
-            if (((bx + by + z) & 1) == 0) {
-                const T *Pm = sp[(z + NS - 1) % NS];
-                const T *Pp = sp[(z + 1) % NS];
-                T aw, ae, as, an, ab, at, ap;
-                coefs<T, VAR>(g, cx, cy, cz, q, bx, by, z, aw, ae, as, an, ab, at, ap);
-                T s = aw * P[l - 1];
-                s = s + ae * P[l + 1];
-                s = s + as * P[l - LX];
-                s = s + an * P[l + LX];
-                s = s + ab * Pm[l];
-                s = s + at * Pp[l];
-                v = (rb_cur - s) / ap;
-            } else {
+            if (blk)
+                v = update(P, sp[(z + NS - 1) % NS], sp[(z + 1) % NS], l, bbnd[pz], z, q, rb_cur,
+                           xy_clean && z_clean(z));
+            else
                 v = P[l];
-            }
             pout[q] = v;
         }
     };
 
     const int c0 = t.z0 - 1, c1 = t.z1 + 1;
-    Regs R0, R1, R2;
+    Regs R0, R1;
     load_regs(R0, c0);
     load_regs(R1, c0 + 1);
-    load_regs(R2, c0 + 2);
     int c = c0;
-    for (; c + 2 <= c1; c += 3) {
+    for (; c + 1 <= c1; c += 2) {
         step(c, R0);
         step(c + 1, R1);
-        step(c + 2, R2);
     }
     if (c <= c1) step(c, R0);
-    if (c + 1 <= c1) step(c + 1, R1);
 }
 
 // Ghost fill from a region table: exchangepp + mgbound + exchbc_mgpp of the

@@ -230,6 +230,10 @@ class PressureMultigrid:
         _lib.lib().mf3d_mg_kernel_time(self._h, ctypes.byref(n), ctypes.byref(ms))
         return int(n.value), float(ms.value)
 
+    def graph_replay(self) -> bool:
+        """Whether the last solve replayed captured cycle graphs."""
+        return bool(_lib.lib().mf3d_mg_graph_replay(self._h))
+
     def solve_device(self, f_dev_ptr: int, p_dev_ptr: int, stream_ptr: int = 0) -> SolveResult:
         """Device-resident solve on interior-packed device vectors (e.g. torch
         tensors' data_ptr()); the fields never leave HBM."""

@@ -110,6 +110,22 @@ def join_blocks(grid: BlockGrid, parts) -> np.ndarray:
     return out
 
 
+UNEVEN_EXTENTS = (96, 128, 160, 192)
+
+
+def uneven_extents(seed: int, nblocks=(2, 2, 2), choices=UNEVEN_EXTENTS):
+    """Per-axis block extents of an uneven block grid: successive SplitMix64
+    draws x of ``seed`` (the x-axis blocks first, then y, then z) pick
+    ``choices[x % len(choices)]``."""
+    x = splitmix64(seed, 0, sum(nblocks))
+    pick = [int(choices[int(v % np.uint64(len(choices)))]) for v in x]
+    out, o = [], 0
+    for k in nblocks:
+        out.append(pick[o:o + k])
+        o += k
+    return out
+
+
 LSM_PI = 3.14159265359   # lsm.for:1466
 
 
