@@ -278,6 +278,23 @@ std::vector<Tile> sweep_tiles(const std::vector<BlockLevel> &hb, int ty) {
     return t;
 }
 
+// The dispatcher hands workgroup i to XCD i mod 8.  Reorder a spatially
+// ordered tile list so that XCD k receives the k-th contiguous eighth:
+// tiles that share halo planes then run under the same L2.  Results do not
+// depend on the order.  MF3D_XCD_ORDER=0 keeps the spatial order.
+std::vector<Tile> xcd_order(const std::vector<Tile> &t) {
+    const char *env = std::getenv("MF3D_XCD_ORDER");
+    if (env && env[0] == '0') return t;
+    constexpr size_t X = 8;
+    const size_t n = t.size(), q = n / X, r = n % X;
+    std::vector<Tile> out(n);
+    for (size_t i = 0; i < n; ++i) {
+        const size_t k = i % X, j = i / X;
+        out[i] = t[k * q + std::min(k, r) + j];
+    }
+    return out;
+}
+
 inline int inner_bytes(const mf3d_mg_cfg &c) { return c.precision == MF3D_MIXED ? 4 : 8; }
 
 long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
@@ -373,14 +390,15 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
     constexpr bool RCP = sizeof(T) == 4;
     const T *pin = (const T *)L.phi[L.cur];
     T *pout = (T *)L.phi[1 - L.cur];
+    const uint32_t bytes = (uint32_t)(L.cells_gh * (long long)sizeof(T));
     if (L.sw_ty == 8)
         hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
-                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db, L.t_sw,
-                           done);
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, L.db,
+                           L.t_sw, done);
     else
         hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
-                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, L.db, L.t_sw,
-                           done);
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, L.db,
+                           L.t_sw, done);
 }
 
 // `sweeps` sweeps at level l; the first treats the potential as zero when
@@ -773,11 +791,17 @@ int build(mf3d_mg *mg) {
             g.gz = B.h[0] * B.h[1] / B.h[2];
         }
         V.cells_gh = off;
+        if (off * (long long)inner_bytes(c) >= (1LL << 32)) {
+            mg->err = "a level holds more than 4 GiB of one field (the sweep addresses arenas with 32-bit "
+                      "byte offsets): use fewer cells per solver";
+            return MF3D_ENOTSUP;
+        }
         // the finest level marches 8-row tiles; coarser levels have few
         // columns and use 4-row tiles, which keeps their z chunks deeper
         V.sw_ty = (l == 0 && n1max >= 8) ? 8 : 4;
         if ((rc = upload(mg, &V.db, V.hb))) return rc;
-        auto tsw = sweep_tiles(V.hb, V.sw_ty), tin = cell_tiles(V.hb), trow = row_tiles(V.hb);
+        auto tsw = xcd_order(sweep_tiles(V.hb, V.sw_ty)), tin = xcd_order(cell_tiles(V.hb)),
+             trow = xcd_order(row_tiles(V.hb));
         V.n_sw = (int)tsw.size(); V.n_in = (int)tin.size(); V.n_row = (int)trow.size();
         if ((rc = upload(mg, &V.t_sw, tsw)) || (rc = upload(mg, &V.t_in, tin)) || (rc = upload(mg, &V.t_row, trow)))
             return rc;
@@ -804,7 +828,7 @@ int build(mf3d_mg *mg) {
     }
     Level &L0 = mg->lv[0];
     {
-        auto tout = sweep_tiles(L0.hb, 8);
+        auto tout = xcd_order(sweep_tiles(L0.hb, 8));
         mg->n_out = (int)tout.size();
         if ((rc = upload(mg, &mg->t_out, tout))) return rc;
     }

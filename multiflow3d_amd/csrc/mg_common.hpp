@@ -106,6 +106,29 @@ __device__ __forceinline__ void coefs(const BlockLevel &g, const T *__restrict__
     ap = -s;   // -1.0*(...) of the reference: negation is exact
 }
 
+// Raw buffer resources (gfx9 layout, word 3 = 0x00020000): loads and
+// stores take a per-thread 32-bit byte offset and a scalar byte offset, and
+// the hardware returns zero for (and drops) accesses at or beyond `bytes`.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <> __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T> __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, T v, uint32_t voff, uint32_t soff);
+template <> __device__ __forceinline__ void bstore<double>(__amdgpu_buffer_rsrc_t r, double v, uint32_t voff, uint32_t soff) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, voff, soff, 0);
+}
+template <> __device__ __forceinline__ void bstore<float>(__amdgpu_buffer_rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), r, voff, soff, 0);
+}
+
 // Fixed-shape tree reduction of (sum, max) over a workgroup of NT threads
 // (a power of two); the order never changes, so results are bitwise
 // reproducible run to run.

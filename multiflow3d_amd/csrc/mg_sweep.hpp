@@ -1,5 +1,7 @@
 // mg_sweep.hpp -- red-black Gauss-Seidel sweep and ghost exchange.
 #pragma once
+#include <type_traits>
+
 #include "mg_common.hpp"
 
 namespace mf3d {
@@ -13,100 +15,115 @@ namespace mf3d {
 // indices, i.e. (i+j+k) odd on the interior indices used here.
 //
 // A workgroup owns SW_TX x TY columns of one block and marches over the
-// planes z0..z1.  LDS keeps a ring of 5 planes of pin, each with a 2-cell
-// halo in x and y.  At step c the first colour is computed on plane c over
-// the tile plus a 1-cell ring (in place in LDS), then the second colour on
-// plane c-1: half the threads compute its cells, the other half copy the
-// first-colour cells of that plane out of LDS.  Red cells in the ring are
-// recomputed by both neighbouring workgroups from the same inputs, so they
-// are bitwise the values the reference computes once.
+// planes z0..z1 (z0 odd: chunks start at 1 and have even depth).  LDS keeps
+// a ring of 5 planes of pin, each with a 2-cell halo in x and y.  At step c
+// the first colour is computed on plane c over the tile plus a 1-cell ring
+// (in place in LDS), then the second colour on plane c-1: half the threads
+// compute its cells, the other half copy the first-colour cells of that
+// plane out of LDS.  Red cells in the ring are recomputed by both
+// neighbouring workgroups from the same inputs, so they are bitwise the
+// values the reference computes once.
 //
-// Loads run ahead of use: two register sets alternate between even and odd
-// steps, so while step c computes, plane c+2 of pin and the RHS of the next
-// cells are in flight; loads are unconditional (clamped address, masked
-// result) so the compiler waits only for the one it needs.  Addresses and
-// LDS indices are precomputed per thread for both plane parities.  Where no
-// Neumann-zeroed coefficient is in reach of the tile (the common case) the
-// constant coefficients and ap are hoisted.
+// The kernel is bound by instruction issue before HBM, so the march keeps
+// scalar and address work out of the loop.  The main loop is unrolled by
+// 10 (the ring has 5 slots and colours alternate), which makes every LDS
+// slot, colour parity and register set compile-time; the leftover steps
+// (an even number) run first with slots computed at run time.  Global
+// traffic goes through buffer resources: a per-thread 32-bit offset and a
+// per-plane scalar offset, no address arithmetic per load; loads are
+// unconditional (clamped in-bounds offsets) and never masked, since values
+// outside the block feed no update.  Two register sets alternate, so plane
+// c+2 and the next RHS values are in flight while step c computes.  Where
+// no Neumann-zeroed coefficient is in reach of the tile the constant
+// coefficients and ap are hoisted.
 // ZERO: pin is identically zero (the first sweep after restriction, where
 // the reference has just zeroed phi, mgsolver.for:650-657).
 // RCP: multiply by 1/ap instead of dividing (inner cycle of the mixed
 // precision form only; the FP64 form divides, as the reference does).
+// `bytes` is the arena length in bytes (below 2^32, checked by the host).
 constexpr int SW_TX = 64;
+constexpr int SW_UNROLL = 10;
+
+template <int J>
+using Step = std::integral_constant<int, J>;
 
 template <typename T, bool VAR, bool ZERO, bool RCP, int TY>
 __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T *__restrict__ pout,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ cx,
                                                   const T *__restrict__ cy,
-                                                  const T *__restrict__ cz,
+                                                  const T *__restrict__ cz, uint32_t bytes,
                                                   const BlockLevel *__restrict__ bls,
                                                   const Tile *__restrict__ tiles,
                                                   const int *__restrict__ done) {
     if (done && *done) return;
     constexpr int LX = SW_TX + 4, LY = TY + 4, NS = 5, NT = SW_TX * TY, LXY = LX * LY;
     constexpr int KL = (LXY + NT - 1) / NT;
+    constexpr int PS = KL * NT;            // slot stride: every thread stores KL values
     constexpr int HALF = (SW_TX + 2) / 2;  // first-colour cells per ring row
     constexpr int NB = NT / 2;             // second-colour cells per tile plane
-    __shared__ T sp[NS][LXY];
+    constexpr uint32_t W = sizeof(T);
+    __shared__ T sp[NS * PS];
     const Tile t = tiles[blockIdx.x];
     const BlockLevel g = bls[t.b];
+    const auto rs_in = make_rsrc(pin, bytes), rs_rhs = make_rsrc(rhs, bytes), rs_out = make_rsrc(pout, bytes);
     const int tid = threadIdx.x + SW_TX * threadIdx.y;
     const int xb = t.x0 - 2, yb = t.y0 - 2;
-    const int zlo = t.z0 - 2 < 0 ? 0 : t.z0 - 2;
     const int zhi = t.z1 + 2 > g.n2 + 1 ? g.n2 + 1 : t.z1 + 2;
-    const long long sz = g.sz;
+    const int sy = (int)g.sy;
+    const uint32_t pb = (uint32_t)g.sz * W;   // plane bytes
+    const uint32_t ob = (uint32_t)g.off * W;  // block offset bytes
 
-    // this thread's share of a plane load (clamped address when outside)
-    long long lofs[KL];
-    bool lok[KL];
+    // this thread's share of a plane (in-bounds offset when outside)
+    uint32_t lofs[KL];
 #pragma unroll
     for (int k = 0; k < KL; ++k) {
         const int p = tid + k * NT;
         const int ly = p / LX, lx = p - ly * LX;
         const int x = xb + lx, y = yb + ly;
-        lok[k] = p < LXY && x >= 0 && x <= g.n0 + 1 && y >= 0 && y <= g.n1 + 1;
-        lofs[k] = lok[k] ? g.off + (long long)x + g.sy * y : g.off + g.sy + 1;
+        const bool ok = p < LXY && x >= 0 && x <= g.n0 + 1 && y >= 0 && y <= g.n1 + 1;
+        lofs[k] = ob + (uint32_t)(ok ? x + sy * y : sy + 1) * W;
     }
 
     // first-colour (red) ring cell of this thread, per plane parity
     const int r_row = tid / HALF, r_m = tid - r_row * HALF;
     const int ry = t.y0 - 1 + r_row, rxs = t.x0 - 1;
     bool rok[2];
-    int rl[2];
-    long long rg[2];
-    int rbnd[2];  // x/y boundary position bits for the coefficients
+    int rl[2], rx[2];
+    uint32_t ro[2];
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
         const int x = rxs + ((1 - ry - par - rxs) & 1) + 2 * r_m;
         rok[par] = tid < HALF * (TY + 2) && x >= 1 && x <= g.n0 && ry >= 1 && ry <= g.n1;
         rl[par] = (x - xb) + LX * (ry - yb);
-        rg[par] = rok[par] ? g.off + (long long)x + g.sy * ry : g.off;
-        rbnd[par] = x | (ry << 16);
+        ro[par] = ob + (uint32_t)(rok[par] ? x + sy * ry : 0) * W;
+        rx[par] = x;
     }
-    // second phase: threads < NB compute second-colour cells, the others
-    // copy first-colour cells of the same plane out of LDS
+    // second phase: threads < NB (whole waves) compute second-colour
+    // cells, the others copy first-colour cells of the same plane
     const bool blk = tid < NB;
     const int bt = blk ? tid : tid - NB;
     const int b_row = bt / (SW_TX / 2), b_m = bt - b_row * (SW_TX / 2);
     const int by = t.y0 + b_row;
     bool bok[2];
-    int bl[2];
-    long long bg[2];
-    int bbnd[2];
+    int bl[2], bx[2];
+    uint32_t bo[2];
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
         const int e = (1 + by + par) & 1;  // offset of the second-colour cell in its pair
         const int x = t.x0 + 2 * b_m + (blk ? e : 1 - e);
         bok[par] = x <= g.n0 && by <= g.n1;
         bl[par] = (x - xb) + LX * (by - yb);
-        bg[par] = bok[par] ? g.off + (long long)x + g.sy * by : g.off;
-        bbnd[par] = x | (by << 16);
+        bo[par] = ob + (uint32_t)(bok[par] ? x + sy * by : 0) * W;
+        bx[par] = x;
     }
 
-    // hoisted constant coefficients where no Neumann zero is in reach
+    // hoisted constant coefficients where no Neumann zero is in reach:
+    // planes zf0..zf1 of this tile
     const bool xy_clean = !VAR && (!(g.neu & kNeuW) || t.x0 > 2) && (!(g.neu & kNeuE) || t.x0 + SW_TX < g.n0) &&
                           (!(g.neu & kNeuS) || t.y0 > 2) && (!(g.neu & kNeuN) || t.y0 + TY < g.n1);
+    const int zf0 = !xy_clean ? 1 << 30 : ((g.neu & kNeuB) ? 2 : -(1 << 30));
+    const int zf1 = (g.neu & kNeuT) ? g.n2 - 1 : 1 << 30;
     const T ca = cconst<T>(g, 0), cb = cconst<T>(g, 1), cc = cconst<T>(g, 2);
     T ap0;
     {
@@ -118,13 +135,9 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
         ap0 = -s0;
     }
     const T rap0 = T(1) / ap0;
-    auto z_clean = [&](int z) {
-        return (!(g.neu & kNeuB) || z > 1) && (!(g.neu & kNeuT) || z < g.n2);
-    };
-    // one Gauss-Seidel update from the LDS planes P (this), Pm, Pp
-    auto update = [&](const T *P, const T *Pm, const T *Pp, int l, int bnd, int z, long long q, T r,
-                      bool fast) -> T {
-        if (fast) {
+    // one Gauss-Seidel update; P, Pm, Pp: this plane and the planes below/above
+    auto update = [&](const T *P, const T *Pm, const T *Pp, int l, int x, int y, int z, uint32_t o, T r) -> T {
+        if (z >= zf0 && z <= zf1) {
             T s = ca * P[l - 1];
             s = s + ca * P[l + 1];
             s = s + cb * P[l - LX];
@@ -134,7 +147,8 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
             return RCP ? (r - s) * rap0 : (r - s) / ap0;
         }
         T aw, ae, as, an, ab, at, ap;
-        coefs<T, VAR>(g, cx, cy, cz, q, bnd & 0xffff, bnd >> 16, z, aw, ae, as, an, ab, at, ap);
+        const long long q = (long long)(o / W) + g.sz * z;
+        coefs<T, VAR>(g, cx, cy, cz, q, x, y, z, aw, ae, as, an, ab, at, ap);
         T s = aw * P[l - 1];
         s = s + ae * P[l + 1];
         s = s + as * P[l - LX];
@@ -149,84 +163,97 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
         T rr, rb;
     };
     // the values step c needs: plane c+1 of pin, the RHS of this thread's red
-    // cell on plane c and of its second-phase cell on plane c-1
-    auto load_regs = [&](Regs &R, int c) {
-        const long long zo = sz * (long long)(c + 1 < zhi ? c + 1 : zhi);
+    // cell on plane c (parity PR) and of its second-phase cell on plane c-1
+    auto load_regs = [&](Regs &R, int c, auto parc) {
+        constexpr int PR = decltype(parc)::value;
+        if (!ZERO) {
+            const uint32_t so = (uint32_t)(c + 1 < zhi ? c + 1 : zhi) * pb;
 #pragma unroll
-        for (int k = 0; k < KL; ++k) {
-            if (ZERO) {
-                R.pre[k] = T(0);
-            } else {
-                const T v = pin[lofs[k] + zo];
-                R.pre[k] = lok[k] ? v : T(0);
-            }
+            for (int k = 0; k < KL; ++k) R.pre[k] = bload<T>(rs_in, lofs[k], so);
         }
-        const int pr = c & 1;
-        const bool okr = rok[pr] && c >= 1 && c <= g.n2;
-        const T v = rhs[(okr ? rg[pr] : g.off) + (okr ? sz * c : 0)];
-        R.rr = okr ? v : T(0);
-        const int pb = (c - 1) & 1;
-        const bool okb = blk && bok[pb] && c - 1 >= t.z0 && c - 1 <= t.z1;
-        const T w = rhs[(okb ? bg[pb] : g.off) + (okb ? sz * (c - 1) : 0)];
-        R.rb = okb ? w : T(0);
+        const int cr = c < 0 ? 0 : (c > g.n2 + 1 ? g.n2 + 1 : c);
+        R.rr = bload<T>(rs_rhs, ro[PR], (uint32_t)cr * pb);
+        const int cb1 = c - 1 < 0 ? 0 : (c - 1 > g.n2 + 1 ? g.n2 + 1 : c - 1);
+        R.rb = bload<T>(rs_rhs, bo[1 - PR], (uint32_t)cb1 * pb);
     };
 
-    // prologue: planes z0-2 and z0-1 straight into LDS
-    for (int c = t.z0 - 2; c < t.z0; ++c)
-        if (c >= zlo) {
-            T *P = sp[c % NS];
-#pragma unroll
-            for (int k = 0; k < KL; ++k) {
-                if (tid + k * NT >= LXY) continue;
-                P[tid + k * NT] = (!ZERO && lok[k]) ? pin[lofs[k] + sz * c] : T(0);
-            }
-        }
+    const int c0 = t.z0 - 1;  // even: chunks start at odd z0
+    int nsteps = t.z1 - t.z0 + 3;
+    nsteps += nsteps & 1;     // a last chunk of odd depth gets one idle step
+    const int rem = nsteps % SW_UNROLL;  // even
+    const int cm = c0 + rem;  // first step of the unrolled loop
+    // plane z lives in slot (z - cm + 2) mod 5
+    auto slot = [&](int z) { return ((z - cm + 2) % NS + NS) % NS; };
 
-    auto step = [&](int c, Regs &R) {
-        {
-            T *P = sp[(c + 1) % NS];  // beyond zhi the slot holds a plane nobody reads
+    // prologue: planes z0-2 and z0-1
 #pragma unroll
-            for (int k = 0; k < KL; ++k)
-                if (tid + k * NT < LXY) P[tid + k * NT] = R.pre[k];
+    for (int d = 0; d < 2; ++d) {
+        const int c = t.z0 - 2 + d;
+        if (c >= 0) {
+            T *P = sp + slot(c) * PS;
+#pragma unroll
+            for (int k = 0; k < KL; ++k) P[tid + k * NT] = ZERO ? T(0) : bload<T>(rs_in, lofs[k], (uint32_t)c * pb);
+        }
+    }
+
+    Regs R0, R1;
+    if (ZERO) {
+#pragma unroll
+        for (int k = 0; k < KL; ++k) R0.pre[k] = R1.pre[k] = T(0);
+    }
+    load_regs(R0, c0, Step<0>{});
+    load_regs(R1, c0 + 1, Step<1>{});
+
+    // one step on plane c with parity PR; s_*: slots of planes c-2 .. c+1
+    auto step = [&](int c, auto parc, int s_m2, int s_m1, int s_c, int s_p1) {
+        constexpr int PR = decltype(parc)::value;
+        Regs &R = PR ? R1 : R0;
+        {
+            T *P = sp + s_p1 * PS;  // beyond zhi the slot holds a plane nobody reads
+#pragma unroll
+            for (int k = 0; k < KL; ++k) P[tid + k * NT] = R.pre[k];
         }
         const T rr_cur = R.rr, rb_cur = R.rb;
-        load_regs(R, c + 2);
+        load_regs(R, c + 2, parc);
         __syncthreads();
-        {
-            const int pr = c & 1;
-            if (rok[pr] && c >= 1 && c <= g.n2) {
-                T *P = sp[c % NS];
-                const int l = rl[pr];
-                P[l] = update(P, sp[(c + NS - 1) % NS], sp[(c + 1) % NS], l, rbnd[pr], c,
-                              rg[pr] + sz * c, rr_cur, xy_clean && z_clean(c));
-            }
+        if (c >= 1 && c <= g.n2 && rok[PR]) {
+            T *P = sp + s_c * PS;
+            const int l = rl[PR];
+            P[l] = update(P, sp + s_m1 * PS, sp + s_p1 * PS, l, rx[PR], ry, c, ro[PR], rr_cur);
         }
         __syncthreads();
-        const int z = c - 1, pz = z & 1;
-        if (bok[pz] && z >= t.z0 && z <= t.z1) {
-            const T *P = sp[z % NS];
-            const int l = bl[pz];
-            const long long q = bg[pz] + sz * z;
+        const int z = c - 1;
+        if (z >= t.z0 && z <= t.z1 && bok[1 - PR]) {
+            const T *P = sp + s_m1 * PS;
+            const int l = bl[1 - PR];
             T v;
             if (blk)
-                v = update(P, sp[(z + NS - 1) % NS], sp[(z + 1) % NS], l, bbnd[pz], z, q, rb_cur,
-                           xy_clean && z_clean(z));
+                v = update(P, sp + s_m2 * PS, sp + s_c * PS, l, bx[1 - PR], by, z, bo[1 - PR], rb_cur);
             else
                 v = P[l];
-            pout[q] = v;
+            bstore<T>(rs_out, v, bo[1 - PR], (uint32_t)z * pb);
         }
     };
 
-    const int c0 = t.z0 - 1, c1 = t.z1 + 1;
-    Regs R0, R1;
-    load_regs(R0, c0);
-    load_regs(R1, c0 + 1);
-    int c = c0;
-    for (; c + 1 <= c1; c += 2) {
-        step(c, R0);
-        step(c + 1, R1);
+    for (int c = c0; c < cm; c += 2) {
+        step(c, Step<0>{}, slot(c - 2), slot(c - 1), slot(c), slot(c + 1));
+        step(c + 1, Step<1>{}, slot(c - 1), slot(c), slot(c + 1), slot(c + 2));
     }
-    if (c <= c1) step(c, R0);
+    // unrolled: step J on plane c = cm + 10m + J keeps plane c in slot (J + 2) mod 5
+#define MF3D_RBS_STEP(J) step(c + J, Step<(J) & 1>{}, (J) % NS, ((J) + 1) % NS, ((J) + 2) % NS, ((J) + 3) % NS)
+    for (int c = cm; c < c0 + nsteps; c += SW_UNROLL) {
+        MF3D_RBS_STEP(0);
+        MF3D_RBS_STEP(1);
+        MF3D_RBS_STEP(2);
+        MF3D_RBS_STEP(3);
+        MF3D_RBS_STEP(4);
+        MF3D_RBS_STEP(5);
+        MF3D_RBS_STEP(6);
+        MF3D_RBS_STEP(7);
+        MF3D_RBS_STEP(8);
+        MF3D_RBS_STEP(9);
+    }
+#undef MF3D_RBS_STEP
 }
 
 // Ghost fill from a region table: exchangepp + mgbound + exchbc_mgpp of the
