@@ -36,6 +36,25 @@ def algorithmic_bytes_per_cycle(n_cells_levels, w, c=0):
     return b_inner + b_outer
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rbs_256cubed_L7_traffic.md")
+
+
+def measured_traffic(form):
+    """HBM bytes per finest-level k_rbs launch of the 256^3, 7-level workload
+    (3 full sweeps + 1 zero-start sweep), from the committed rocprofv3 PMC
+    summary: FETCH_SIZE x 2 + WRITE_SIZE, the factors fixed by a 1 GiB copy
+    in the same runs.  None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    kv = dict(item.split("=") for item in line.split(":", 1)[1].split())
+                    return float(kv[form])
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
+
+
 def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_kernel):
     import torch
 
@@ -148,8 +167,12 @@ def main():
         avg_s = r["kernel_ms"] / r["launches"] * 1e-3
         per_launch = 2.75 * w * N0
         achieved = per_launch / avg_s / 1e9
+        traffic = measured_traffic("fp64" if w == 8 else "mixed") if (n, L) == (256, 7) else None
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": traffic,
+                "traffic_source": "profiles/k_rbs_256cubed_L7_traffic.md (rocprofv3 FETCH_SIZE x2 + "
+                                  "WRITE_SIZE per launch, calibrated on a 1 GiB copy)" if traffic else None,
                 "kernel": "k_rbs (fused red-black sweep, finest level)", "avg_us": round(avg_s * 1e6, 2),
                 "launches_timed": r["launches"], "bytes_per_launch": int(per_launch),
                 "timed_with": "HIP events on the solver's stream around each launch, inside the timed "
