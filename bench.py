@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+W_SEED = 20261202  # the workloads' base seed
 METRIC = "MG solves/s to 1e-8 resid (mixed vs FP64), HBM GB/s % peak, p50/p99 ms"
 
 
@@ -64,6 +65,7 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     dev = torch.device("cuda:0")
     f_dev = torch.from_numpy(np.ascontiguousarray(np.concatenate([a.ravel() for a in f_host]))).to(dev)
     p_dev = torch.zeros_like(f_dev)
+    torch.cuda.synchronize()  # the solver runs on its own non-blocking stream
     mg = PressureMultigrid(grid, levels, maxcy=maxcy, eps=tol, precision=precision)
     if time_kernel:
         mg.time_kernel(_lib.MF3D_K_SMOOTH)
@@ -93,6 +95,35 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     return out
 
 
+def run_serving(count):
+    """Serving sweep: Poisson arrivals of independent solves, n^3 with n
+    uniform in {32,64,96,128,192,256}, tolerance uniform in {1e-6,1e-8,1e-10},
+    constant or two-phase coefficient 50/50, mixed precision, 4 slots
+    (streams), hierarchies resident after one warm-up request per shape."""
+    from multiflow3d_amd import serving as S
+    srv = S.Server(slots=4, precision="mixed")
+    out = {"slots": 4, "precision": "mixed", "requests_per_rate": count, "rates": []}
+    try:
+        warm = [S.Request(i, 0.0, n, 1e-6, tp, W_SEED + 5) for i, (n, tp) in
+                enumerate((n, tp) for n in S.SIZES for tp in (False, True))]
+        srv.run(warm, realtime=False)
+        for rate in (100.0, 400.0, 1600.0):
+            reqs = S.poisson_requests(rate, count, W_SEED + 50)
+            res = srv.run(reqs)
+            summ = S.summarize(res, srv.elapsed)
+            summ["rate_per_s"] = rate
+            out["rates"].append(summ)
+            print(f"# serving rate {rate:g}/s: {summ}", file=sys.stderr, flush=True)
+        reqs = S.poisson_requests(1.0, count, W_SEED + 50)
+        res = srv.run(reqs, realtime=False)
+        out["closed_batch"] = S.summarize(res, srv.elapsed)
+        out["pool_peak_bytes"] = srv.pool.peak
+        out["handles_created"] = srv.pool.created
+    finally:
+        srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +134,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxcy", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=64,
+                    help="requests per arrival rate in the serving sweep (0: skip it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device-to-device once (a copy of known size "
                          "to scale HBM byte counters read by a profiler)")
@@ -200,6 +233,8 @@ def main():
                          f"single thread",
                "fp64_gpu_bitwise_equal": fp64_bitwise}
 
+    serving = run_serving(args.serving_requests) if args.serving_requests > 0 else None
+
     lat = np.array(rmx["lat"])
     line = {
         "metric": METRIC,
@@ -233,6 +268,7 @@ def main():
                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
                   "rel_l2_vs_fp64": mixed_err, "graph_replay": rmx["graph_replay"]},
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MF3D_MG_ABI_VERSION 1
+#define MF3D_MG_ABI_VERSION 2
 
 enum {
     MF3D_OK = 0,
@@ -45,6 +45,22 @@ enum {
 };
 
 enum { MF3D_FP64 = 0, MF3D_MIXED = 1 };
+
+/* Coarse operators of the variable-coefficient form (mf3d_mg_cfg.coarse_op):
+ * the reference's density hierarchy (8-cell density average, face
+ * coefficients from the finest dx, dy, dz on every level, mgsolver.for:69-181),
+ * or face-coefficient restriction (each coarse face coefficient is the sum of
+ * the four fine face coefficients it covers / 16, i.e. the mean flux
+ * coefficient rescaled to the coarse spacing).  No counterpart of the second
+ * in the reference: with a 1000:1 density ratio the first stalls. */
+enum { MF3D_COARSE_REFERENCE = 0, MF3D_COARSE_FACE = 1 };
+
+/* Outer iteration (mf3d_mg_cfg.krylov): the reference's defect correction
+ * p += V(f - A p) (newsolv_mg.for:23-58), or GCR(1) with the V-cycle as
+ * preconditioner (residual-minimising, one direction kept, valid for the
+ * reference's non-symmetric multi-block density operator; no counterpart in
+ * the reference). */
+enum { MF3D_OUTER_DEFECT = 0, MF3D_OUTER_GCR = 1 };
 
 typedef struct mf3d_mg mf3d_mg;
 
@@ -75,6 +91,8 @@ typedef struct {
     int32_t precision;     /* MF3D_FP64 (bitwise the reference) or MF3D_MIXED */
     int32_t variable_coef; /* 0: 1/h^2 coefficients; 1: LSM density coefficients */
     int32_t device;        /* HIP device ordinal */
+    int32_t coarse_op;     /* MF3D_COARSE_*; variable-coefficient form only */
+    int32_t krylov;        /* MF3D_OUTER_* */
 } mf3d_mg_cfg;
 
 /* coeff sizing + constant coefficients (mgsolver.for:2-222, localparameters.for:200-237). */
@@ -127,6 +145,11 @@ int64_t mf3d_mg_estimate_bytes(const mf3d_mg_cfg *cfg, const mf3d_block_desc *bl
 enum { MF3D_K_OFF = 0, MF3D_K_SMOOTH = 1, MF3D_K_OUTER = 2, MF3D_K_RESTRICT = 3, MF3D_K_PROLONG = 4 };
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel);
 int mf3d_mg_kernel_time(mf3d_mg *mg, int64_t *launches, double *total_ms);
+
+/* Stop tolerance of the following solves (the eps of mf3d_mg_cfg), so one
+ * resident handle serves requests of different tolerances.  No
+ * counterpart in the reference, whose eps is a global of control.cin. */
+int mf3d_mg_set_eps(mf3d_mg *mg, double eps);
 
 /* 1 if the last solve replayed captured cycle graphs, 0 if it launched its
  * kernels one by one (no counterpart in the reference; for benchmarks). */

@@ -21,8 +21,10 @@ EXPORTS = (
     "mf3d_mg_solve_device", "mf3d_mg_cells", "mf3d_mg_device_bytes",
     "mf3d_mg_estimate_bytes", "mf3d_mg_last_error", "mf3d_mg_destroy",
     "mf3d_mg_abi_version", "mf3d_mg_time_kernel", "mf3d_mg_kernel_time",
-    "mf3d_mg_graph_replay",
+    "mf3d_mg_graph_replay", "mf3d_mg_set_eps",
 )
+MF3D_COARSE_REFERENCE, MF3D_COARSE_FACE = 0, 1
+MF3D_OUTER_DEFECT, MF3D_OUTER_GCR = 0, 1
 MF3D_K_OFF, MF3D_K_SMOOTH, MF3D_K_OUTER, MF3D_K_RESTRICT, MF3D_K_PROLONG = 0, 1, 2, 3, 4
 
 
@@ -39,7 +41,8 @@ class Cfg(ctypes.Structure):
                 ("nu_post", ctypes.c_int32), ("maxcy", ctypes.c_int32),
                 ("eps", ctypes.c_double), ("min_cycles", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("variable_coef", ctypes.c_int32),
-                ("device", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("coarse_op", ctypes.c_int32),
+                ("krylov", ctypes.c_int32)]
 
 
 _lib = None
@@ -88,5 +91,7 @@ def lib():
     L.mf3d_mg_kernel_time.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
     L.mf3d_mg_graph_replay.restype = ctypes.c_int
     L.mf3d_mg_graph_replay.argtypes = [vp]
+    L.mf3d_mg_set_eps.restype = ctypes.c_int
+    L.mf3d_mg_set_eps.argtypes = [vp, ctypes.c_double]
     _lib = L
     return L

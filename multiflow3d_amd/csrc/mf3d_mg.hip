@@ -67,6 +67,9 @@ struct mf3d_mg {
     std::vector<Level> lv;
     double *pbuf[2] = {nullptr, nullptr};
     double *f_g = nullptr;
+    // GCR outer iteration (krylov): directions (ping-pong), w = A d, FP64 residual
+    double *kd[2] = {nullptr, nullptr};
+    double *kw = nullptr, *kr = nullptr;
     double *partials = nullptr;
     int nparts = 0;
     SolveState *st = nullptr;
@@ -139,7 +142,10 @@ template <typename X>
 int upload(mf3d_mg *mg, X **dst, const std::vector<X> &src) {
     int rc = dev_alloc(mg, (void **)dst, sizeof(X) * std::max<size_t>(src.size(), 1));
     if (rc) return rc;
-    if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), sizeof(X) * src.size(), hipMemcpyHostToDevice));
+    if (!src.empty()) {  // on the handle's own stream: never the legacy stream
+        HIPCHK(hipMemcpyAsync(*dst, src.data(), sizeof(X) * src.size(), hipMemcpyHostToDevice, mg->stream));
+        HIPCHK(hipStreamSynchronize(mg->stream));
+    }
     return 0;
 }
 
@@ -155,6 +161,8 @@ std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (c.smoother != 3) return "only the red-black smoother (mg_itrsch = 3) is supported";
     if (c.nu_pre < 0 || c.nu_post < 0 || c.maxcy < 0) return "negative sweep or cycle count";
     if (c.precision != MF3D_FP64 && c.precision != MF3D_MIXED) return "unknown precision";
+    if (c.coarse_op != MF3D_COARSE_REFERENCE && c.coarse_op != MF3D_COARSE_FACE) return "unknown coarse_op";
+    if (c.krylov != MF3D_OUTER_DEFECT && c.krylov != MF3D_OUTER_GCR) return "unknown outer iteration (krylov)";
     const int div = 1 << (c.levels - 1);
     for (int i = 0; i < c.nblocks; ++i) {
         for (int d = 0; d < 3; ++d) {
@@ -313,6 +321,7 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
         in0 += (long long)b[i].n[0] * b[i].n[1] * b[i].n[2];
     }
     tot += gh0 * 8 * 3;             // p (two buffers) and f
+    if (c.krylov == MF3D_OUTER_GCR) tot += gh0 * 8 * 4;  // d (two buffers), w, r
     tot += std::max(gh0, in0) * 8;  // I/O staging
     return tot;
 }
@@ -489,18 +498,45 @@ void vcycle_dev(mf3d_mg *mg, bool zero_start, const int *done) {
     relax<T, VAR>(mg, 0, mg->cfg.nu_post, false, done);
 }
 
+// Outer step after the V-cycle of cycle k (from = k mod 2; k = 0: the
+// initial residual, first = true).  Defect correction: p += e, r = f - A p.
+// GCR(1): beta from (A z, w_prev), d = z - beta d_prev, w = A d, alpha,
+// p += alpha d, r = f - A p.
 template <typename T, bool VAR>
-void outer_step(mf3d_mg *mg, int from, const int *done) {
+void outer_step(mf3d_mg *mg, int from, const int *done, bool first = false) {
     Level &L0 = mg->lv[0];
     const double *cx = VAR ? L0.dcx : nullptr, *cy = VAR ? L0.dcy : nullptr, *cz = VAR ? L0.dcz : nullptr;
     const bool tk = tk_on(mg, MF3D_K_OUTER, 0);
+    if (mg->cfg.krylov == MF3D_OUTER_GCR) {
+        const T *z = (const T *)L0.phi[L0.cur];
+        if (!first) {
+            hipLaunchKernelGGL((k_krylov_lds<2, T, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, z,
+                               nullptr, nullptr, nullptr, mg->kw, nullptr, nullptr, cx, cy, cz, mg->st, L0.db,
+                               mg->t_out, mg->partials, done);
+            hipLaunchKernelGGL(k_gcr_scalar<true>, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts,
+                               mg->st);
+            hipLaunchKernelGGL((k_krylov_lds<1, T, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, z,
+                               mg->kd[from], mg->kd[1 - from], nullptr, mg->kr, mg->kw, nullptr, cx, cy, cz, mg->st,
+                               L0.db, mg->t_out, mg->partials, done);
+            hipLaunchKernelGGL(k_gcr_scalar<false>, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts,
+                               mg->st);
+        }
+        if (tk) tk_mark(mg);
+        hipLaunchKernelGGL((k_krylov_lds<0, double, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream,
+                           mg->pbuf[from], mg->kd[1 - from], mg->pbuf[1 - from], mg->f_g, nullptr, mg->kr,
+                           (T *)L0.rhs, cx, cy, cz, mg->st, L0.db, mg->t_out, mg->partials, done);
+        if (tk) tk_mark(mg);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts, mg->st,
+                           mg->d_hist, mg->d_rmax, mg->cfg.min_cycles);
+        return;
+    }
     if (tk) tk_mark(mg);
     hipLaunchKernelGGL((k_outer_lds<T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, mg->pbuf[from],
                        mg->pbuf[1 - from], (const T *)L0.phi[L0.cur], mg->f_g, (T *)L0.rhs, cx, cy, cz, L0.db,
                        mg->t_out, mg->partials, done);
     if (tk) tk_mark(mg);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts, mg->st,
-                       mg->d_hist, mg->d_rmax, mg->cfg.eps, mg->cfg.min_cycles);
+                       mg->d_hist, mg->d_rmax, mg->cfg.min_cycles);
 }
 
 // Record one cycle reading pbuf[g] into a graph.
@@ -508,7 +544,7 @@ template <typename T, bool VAR>
 int capture_cycles(mf3d_mg *mg) {
     for (int g = 0; g < 2; ++g) {
         for (Level &V : mg->lv) V.cur = 0;
-        HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeRelaxed));
         vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
         outer_step<T, VAR>(mg, g, done_ptr(mg));
         hipGraph_t graph = nullptr;
@@ -544,7 +580,7 @@ int capture_timed(mf3d_mg *mg, int k) {
     }
     mg->ev_rec = &mg->tkev[k];
     mg->ev_fail = false;
-    HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
+    HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeRelaxed));
     vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
     outer_step<T, VAR>(mg, k % 2, done_ptr(mg));
     hipGraph_t graph = nullptr;
@@ -586,13 +622,18 @@ int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rma
             timed = false;
         }
     mg->last_replay = use_graph || timed;
-    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st);
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st, mg->cfg.eps);
     fill<double>(mg, mg->pbuf[0], L0.cells_gh);
     fill<T>(mg, (T *)L0.phi[L0.cur], L0.cells_gh);
+    if (mg->cfg.krylov == MF3D_OUTER_GCR) {
+        fill<double>(mg, mg->kd[0], L0.cells_gh);
+        fill<double>(mg, mg->kd[1], L0.cells_gh);
+        fill<double>(mg, mg->kw, L0.cells_gh);
+    }
     hipLaunchKernelGGL(k_unpack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, f_packed, mg->f_g,
                        L0.db, L0.t_in);
     // step 0: r_0 = f - A*0 from pbuf[0] into pbuf[1]
-    outer_step<T, VAR>(mg, 0, nullptr);
+    outer_step<T, VAR>(mg, 0, nullptr, true);
     int launched = 0;
     SolveState hs{};
     hs.done = 0;
@@ -837,6 +878,12 @@ int build(mf3d_mg *mg) {
         (rc = dev_alloc(mg, (void **)&mg->pbuf[1], L0.cells_gh * 8)) ||
         (rc = dev_alloc(mg, (void **)&mg->f_g, L0.cells_gh * 8)))
         return rc;
+    if (c.krylov == MF3D_OUTER_GCR &&
+        ((rc = dev_alloc(mg, (void **)&mg->kd[0], L0.cells_gh * 8)) ||
+         (rc = dev_alloc(mg, (void **)&mg->kd[1], L0.cells_gh * 8)) ||
+         (rc = dev_alloc(mg, (void **)&mg->kw, L0.cells_gh * 8)) ||
+         (rc = dev_alloc(mg, (void **)&mg->kr, L0.cells_gh * 8))))
+        return rc;
     mg->nparts = mg->n_out;
     if ((rc = dev_alloc(mg, (void **)&mg->partials, sizeof(double) * 2 * (size_t)mg->nparts)) ||
         (rc = dev_alloc(mg, (void **)&mg->st, sizeof(SolveState))) ||
@@ -863,7 +910,7 @@ int build(mf3d_mg *mg) {
     }
     fill<double>(mg, mg->pbuf[1], L0.cells_gh);
     fill<double>(mg, mg->f_g, L0.cells_gh);
-    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st);
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st, mg->cfg.eps);
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
     return 0;
@@ -872,15 +919,22 @@ int build(mf3d_mg *mg) {
 // coeff, L_LSM branch, from level-0 densities already in lv[0].dens.
 int build_lsm(mf3d_mg *mg) {
     const int L = mg->cfg.levels;
-    for (int l = 1; l < L; ++l) {
+    const bool face = mg->cfg.coarse_op == MF3D_COARSE_FACE;
+    for (int l = 1; l < L && !face; ++l) {
         Level &C = mg->lv[l], &F = mg->lv[l - 1];
         hipLaunchKernelGGL(k_dens_restrict, dim3(C.n_in), dim3(kTileX, kTileY), 0, mg->stream, F.dens, F.db,
                            C.dens, C.db, C.t_in);
     }
     for (int l = 0; l < L; ++l) {
         Level &V = mg->lv[l];
-        hipLaunchKernelGGL(k_lsm_coef, dim3(V.n_row), dim3(kTileX, kTileY), 0, mg->stream, V.dens, V.dcx, V.dcy,
-                           V.dcz, V.db, V.t_row);
+        if (face && l > 0) {
+            const Level &F = mg->lv[l - 1];
+            hipLaunchKernelGGL(k_face_restrict, dim3(V.n_row), dim3(kTileX, kTileY), 0, mg->stream, F.dcx, F.dcy,
+                               F.dcz, F.db, V.dcx, V.dcy, V.dcz, V.db, V.t_row);
+        } else {
+            hipLaunchKernelGGL(k_lsm_coef, dim3(V.n_row), dim3(kTileX, kTileY), 0, mg->stream, V.dens, V.dcx,
+                               V.dcy, V.dcz, V.db, V.t_row);
+        }
         if (mg->cfg.precision == MF3D_MIXED) {
             convert<float, double>(mg, (float *)V.cx, V.dcx, V.cells_gh);
             convert<float, double>(mg, (float *)V.cy, V.dcy, V.cells_gh);
@@ -893,7 +947,11 @@ int build_lsm(mf3d_mg *mg) {
     return 0;
 }
 
+// At every entry point: select the device and drop an error left in this
+// thread's HIP state by unrelated earlier calls, so HIPCHK(hipGetLastError())
+// below reports only this call's own failures.
 int set_device(mf3d_mg *mg) {
+    (void)hipGetLastError();
     HIPCHK(hipSetDevice(mg->cfg.device));
     return 0;
 }
@@ -955,9 +1013,11 @@ int mf3d_mg_create(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, mf3d_m
     mg->blk.assign(blocks, blocks + cfg->nblocks);
     int rc = set_device(mg);
     if (rc) return rc;
-    hipError_t e = hipStreamCreate(&mg->stream);
+    // non-blocking: the handle's work never waits on (or makes the legacy
+    // stream wait on) other threads' streams, some of which may be capturing
+    hipError_t e = hipStreamCreateWithFlags(&mg->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        mg->err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        mg->err = std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e);
         return MF3D_EDEVICE;
     }
     return build(mg);
@@ -1066,6 +1126,12 @@ int mf3d_mg_kernel_time(mf3d_mg *mg, int64_t *launches, double *total_ms) {
     if (launches) *launches = mg->tk_launches;
     if (total_ms) *total_ms = mg->tk_ms;
     return 0;
+}
+
+int mf3d_mg_set_eps(mf3d_mg *mg, double eps) {
+    if (!mg || !(eps >= 0.0) || !std::isfinite(eps)) return MF3D_EINVAL;
+    mg->cfg.eps = eps;  // read by the next solve (captured cycles take it from device memory)
+    return MF3D_OK;
 }
 
 int mf3d_mg_graph_replay(const mf3d_mg *mg) { return mg ? mg->last_replay : MF3D_EINVAL; }

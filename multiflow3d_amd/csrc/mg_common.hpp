@@ -58,7 +58,9 @@ struct SolveState {
     int done;   // converged: later cycle kernels return at once
     int k;      // index of the next outer step
     int last;   // last outer step recorded
-    int pad;
+    int fk;     // GCR steps taken
+    double tol; // stop tolerance of this solve (read by captured cycles)
+    double alpha, beta, gamma;  // GCR: step, direction weight, (w, w)
 };
 
 __device__ __forceinline__ long long gidx(const BlockLevel &g, int i, int j, int k) {
@@ -127,6 +129,27 @@ template <> __device__ __forceinline__ void bstore<double>(__amdgpu_buffer_rsrc_
 template <> __device__ __forceinline__ void bstore<float>(__amdgpu_buffer_rsrc_t r, float v, uint32_t voff, uint32_t soff) {
     using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), r, voff, soff, 0);
+}
+
+// Fixed-shape tree reduction of (sum, sum) over a workgroup of NT threads.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double a, double b, double *out) {
+    __shared__ double sa[NT], sb[NT];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sa[t] = sa[t] + sa[t + w];
+            sb[t] = sb[t] + sb[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = sa[0];
+        out[1] = sb[0];
+    }
 }
 
 // Fixed-shape tree reduction of (sum, max) over a workgroup of NT threads

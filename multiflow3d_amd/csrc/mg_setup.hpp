@@ -38,6 +38,52 @@ __global__ __launch_bounds__(256) void k_dens_restrict(const double *__restrict_
     dc[gidx(gc, i, j, k)] = 0.125 * s;
 }
 
+// Face-coefficient restriction (coarse_op = MF3D_COARSE_FACE; no
+// counterpart in the reference): the coarse x face west of coarse cell
+// (i,j,k) covers the fine x faces west of fine cells (2i-1, 2j-1..2j,
+// 2k-1..2k); its coefficient is their sum / 16, summed with the lower
+// tangential axis fastest.  i runs to n0+1 (the east face of the last
+// cell), likewise j and k for the y and z faces, so the one-sided block-edge
+// faces and the zero Neumann faces carry over.  Row tiles over the coarse
+// ghosted range.
+__global__ __launch_bounds__(256) void k_face_restrict(const double *__restrict__ fcx,
+                                                       const double *__restrict__ fcy,
+                                                       const double *__restrict__ fcz,
+                                                       const BlockLevel *__restrict__ blf,
+                                                       double *__restrict__ ccx, double *__restrict__ ccy,
+                                                       double *__restrict__ ccz,
+                                                       const BlockLevel *__restrict__ blc,
+                                                       const Tile *__restrict__ tiles) {
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel gc = blc[t.b];
+    const int j = t.y0 + threadIdx.y, k = t.z0;
+    if (j > gc.n1 + 1) return;
+    const BlockLevel gf = blf[t.b];
+    for (int i = threadIdx.x; i <= gc.n0 + 1; i += blockDim.x) {
+        const long long q = gidx(gc, i, j, k);
+        const bool jin = j >= 1 && j <= gc.n1, kin = k >= 1 && k <= gc.n2, iin = i >= 1 && i <= gc.n0;
+        const long long p = gidx(gf, 2 * i - 1, 2 * j - 1, 2 * k - 1);
+        if (jin && kin && i >= 1) {
+            double s = fcx[p] + fcx[p + gf.sy];
+            s = s + fcx[p + gf.sz];
+            s = s + fcx[p + gf.sy + gf.sz];
+            ccx[q] = s * 0.0625;
+        }
+        if (iin && kin && j >= 1) {
+            double s = fcy[p] + fcy[p + 1];
+            s = s + fcy[p + gf.sz];
+            s = s + fcy[p + 1 + gf.sz];
+            ccy[q] = s * 0.0625;
+        }
+        if (iin && jin && k >= 1) {
+            double s = fcz[p] + fcz[p + 1];
+            s = s + fcz[p + gf.sy];
+            s = s + fcz[p + 1 + gf.sy];
+            ccz[q] = s * 0.0625;
+        }
+    }
+}
+
 // LSM face coefficients (mgsolver.for:120-203) on one level: cx at
 // (i,j,k), i = 1..n0, is aw of cell i = (dy*dz/dx)/rho_face with rho_face
 // the mean of the two cells (the cell itself at the block edge); cx at

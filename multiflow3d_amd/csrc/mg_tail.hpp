@@ -13,7 +13,7 @@
 namespace mf3d {
 
 constexpr int TAIL_MAXL = 16;
-constexpr int TAIL_NT = 512;
+constexpr int TAIL_NT = 1024;
 
 template <typename T>
 struct TailArgs {
@@ -58,16 +58,21 @@ struct Walk {
     }
 };
 
+// Cells of one colour only: row (j,k) holds the cells i = i0, i0+2, ...
+// with i0 = 1 + ((1 + j + k + parity) & 1); the items of all rows are
+// walked as (row, m) with m < half = (n0 + 1) / 2.
 template <typename T, bool VAR>
 __device__ __forceinline__ void tail_colour(T *sm, const TLv v, const BlockLevel &g, const T *cx,
                                             const T *cy, const T *cz, int parity) {
     T *P = sm + v.po;
     const T *R = sm + v.ro;
-    const int N = v.n0 * v.n1 * v.n2;
-    Walk w(threadIdx.x, TAIL_NT, v.n0, v.n1);
-    for (int c = threadIdx.x; c < N; c += TAIL_NT, w.next(v.n0, v.n1)) {
-        const int i = w.i + 1, j = w.j + 1, k = w.k + 1;
-        if (((i + j + k) & 1) != parity) continue;
+    const int half = (v.n0 + 1) >> 1;
+    const int N = half * v.n1 * v.n2;
+    Walk w(threadIdx.x, TAIL_NT, half, v.n1);
+    for (int c = threadIdx.x; c < N; c += TAIL_NT, w.next(half, v.n1)) {
+        const int j = w.j + 1, k = w.k + 1;
+        const int i = 1 + ((1 + j + k + parity) & 1) + 2 * w.i;
+        if (i > v.n0) continue;
         const int q = i + v.sy * j + v.sz * k;
         T aw, ae, as, an, ab, at, ap;
         coefs<T, VAR>(g, cx, cy, cz, g.off + q, i, j, k, aw, ae, as, an, ab, at, ap);
@@ -83,15 +88,33 @@ __device__ __forceinline__ void tail_colour(T *sm, const TLv v, const BlockLevel
 
 // Every ghost takes the interior cell it stands for: wrap on a periodic
 // axis, reflect on a Neumann one.
+// Only ghost cells are visited: the two z planes (whole ghosted planes),
+// then the two y rows of the interior planes, then the two x columns of the
+// interior rows.  Sources are interior cells, so the order is free.
 template <typename T>
 __device__ __forceinline__ void tail_exchange(T *sm, const TLv v) {
     T *P = sm + v.po;
-    const int a = v.n0 + 2, b = v.n1 + 2, M = a * b * (v.n2 + 2);
-    Walk w(threadIdx.x, TAIL_NT, a, b);
-    for (int c = threadIdx.x; c < M; c += TAIL_NT, w.next(a, b)) {
-        const int i = w.i, j = w.j, k = w.k;
-        const bool gi = i == 0 || i == v.n0 + 1, gj = j == 0 || j == v.n1 + 1, gk = k == 0 || k == v.n2 + 1;
-        if (!(gi || gj || gk)) continue;
+    const int a = v.n0 + 2, b = v.n1 + 2;
+    const int G1 = 2 * a * b, G2 = G1 + 2 * a * v.n2, G3 = G2 + 2 * v.n1 * v.n2;
+    for (int t = threadIdx.x; t < G3; t += TAIL_NT) {
+        int i, j, k;
+        if (t < G1) {
+            const int u = t < a * b ? t : t - a * b;
+            k = t < a * b ? 0 : v.n2 + 1;
+            j = u / a;
+            i = u - j * a;
+        } else if (t < G2) {
+            const int u0 = t - G1, side = u0 >= a * v.n2, u = side ? u0 - a * v.n2 : u0;
+            j = side ? v.n1 + 1 : 0;
+            k = 1 + u / a;
+            i = u - (k - 1) * a;
+        } else {
+            const int u0 = t - G2, side = u0 >= v.n1 * v.n2, u = side ? u0 - v.n1 * v.n2 : u0;
+            i = side ? v.n0 + 1 : 0;
+            k = 1 + u / v.n1;
+            j = 1 + u - (k - 1) * v.n1;
+        }
+        const int c = i + v.sy * j + v.sz * k;
         int si = i, sj = j, sk = k;
         if (i == 0) si = (v.neu & kNeuW) ? 1 : v.n0;
         if (i == v.n0 + 1) si = (v.neu & kNeuE) ? v.n0 : 1;

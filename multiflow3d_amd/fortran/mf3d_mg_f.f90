@@ -12,6 +12,8 @@ module mf3d_mg_f
 
   integer(c_int), parameter, public :: MF3D_OK = 0, MF3D_NOT_CONVERGED = 1, MF3D_DIVERGED = 2
   integer(c_int), parameter, public :: MF3D_FP64 = 0, MF3D_MIXED = 1
+  integer(c_int), parameter, public :: MF3D_COARSE_REFERENCE = 0, MF3D_COARSE_FACE = 1
+  integer(c_int), parameter, public :: MF3D_OUTER_DEFECT = 0, MF3D_OUTER_GCR = 1
 
   ! one dom(ib) (module_multidata.for:16-28,36,116)
   type, bind(c), public :: mf3d_block_desc
@@ -29,6 +31,7 @@ module mf3d_mg_f
     integer(c_int32_t) :: nblocks, levels, smoother, nu_pre, nu_post, maxcy
     real(c_double)     :: eps
     integer(c_int32_t) :: min_cycles, precision, variable_coef, device
+    integer(c_int32_t) :: coarse_op = 0, krylov = 0  ! reference hierarchy, defect correction
   end type
 
   public :: mf3d_mg_create, mf3d_mg_set_density, mf3d_mg_vcycle, mf3d_mg_solve

@@ -51,6 +51,7 @@ class SolveResult:
 
 
 PRECISIONS = {"fp64": _lib.MF3D_FP64, "mixed": _lib.MF3D_MIXED}
+COARSE_OPS = {"reference": _lib.MF3D_COARSE_REFERENCE, "face": _lib.MF3D_COARSE_FACE}
 
 
 def _descs(blocks: Sequence[Block], ghost: int):
@@ -67,7 +68,8 @@ def _descs(blocks: Sequence[Block], ghost: int):
 
 
 def make_cfg(nblocks: int, levels: int, *, nu_pre=2, nu_post=2, maxcy=30, eps=1e-8,
-             min_cycles=1, precision="fp64", variable_coef=False, device=0, smoother=3):
+             min_cycles=1, precision="fp64", variable_coef=False, device=0, smoother=3,
+             coarse_op="reference", krylov=False):
     c = _lib.Cfg()
     c.nblocks, c.levels, c.smoother = nblocks, levels, smoother
     c.nu_pre, c.nu_post, c.maxcy = nu_pre, nu_post, maxcy
@@ -75,6 +77,8 @@ def make_cfg(nblocks: int, levels: int, *, nu_pre=2, nu_post=2, maxcy=30, eps=1e
     c.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     c.variable_coef = int(bool(variable_coef))
     c.device = device
+    c.coarse_op = COARSE_OPS[coarse_op] if isinstance(coarse_op, str) else int(coarse_op)
+    c.krylov = int(bool(krylov))
     return c
 
 
@@ -95,7 +99,7 @@ class PressureMultigrid:
     def __init__(self, grid_or_blocks, levels: int, *, nu_pre: int = 2, nu_post: int = 2,
                  maxcy: int = 30, eps: float = 1e-8, min_cycles: int = 1,
                  precision: str = "fp64", variable_coef: bool = False, device: int = 0,
-                 smoother: int = 3):
+                 smoother: int = 3, coarse_op: str = "reference", krylov: bool = False):
         self.blocks: List[Block] = (grid_or_blocks.blocks if isinstance(grid_or_blocks, BlockGrid)
                                     else list(grid_or_blocks))
         self.levels = levels
@@ -103,7 +107,8 @@ class PressureMultigrid:
         self.variable_coef = variable_coef
         self.cfg = make_cfg(len(self.blocks), levels, nu_pre=nu_pre, nu_post=nu_post,
                             maxcy=maxcy, eps=eps, min_cycles=min_cycles, precision=precision,
-                            variable_coef=variable_coef, device=device, smoother=smoother)
+                            variable_coef=variable_coef, device=device, smoother=smoother,
+                            coarse_op=coarse_op, krylov=krylov)
         L = _lib.lib()
         h = ctypes.c_void_p()
         rc = L.mf3d_mg_create(ctypes.byref(self.cfg), _descs(self.blocks, self.GHOST), ctypes.byref(h))
@@ -186,14 +191,14 @@ class PressureMultigrid:
             raise MGError(f"mgkcyc failed ({rc}): {self._err()}")
         return pp
 
-    def _finish(self, rc, rmax, iters):
+    def _finish(self, rc, rmax, iters, warn=True):
         hist = self._hist[: iters.value + 1].copy()
         res = SolveResult(status=rc, rmax=rmax.value, iters=iters.value, hist=hist)
         if rc == _lib.MF3D_DIVERGED:
             raise Diverged(f"BIG RMAX!! STOP!!!!! rmax={rmax.value!r}")
         if rc < 0:
             raise MGError(f"solve failed ({rc}): {self._err()}")
-        if rc == _lib.MF3D_NOT_CONVERGED:
+        if rc == _lib.MF3D_NOT_CONVERGED and warn:
             warnings.warn(f"not converged!! {self.maxcy} {rmax.value!r}", NotConverged)
         return res
 
@@ -230,13 +235,21 @@ class PressureMultigrid:
         _lib.lib().mf3d_mg_kernel_time(self._h, ctypes.byref(n), ctypes.byref(ms))
         return int(n.value), float(ms.value)
 
+    def set_eps(self, eps: float):
+        """Stop tolerance of the following solves."""
+        rc = _lib.lib().mf3d_mg_set_eps(self._h, float(eps))
+        if rc != 0:
+            raise MGError(f"set_eps failed ({rc})")
+
     def graph_replay(self) -> bool:
         """Whether the last solve replayed captured cycle graphs."""
         return bool(_lib.lib().mf3d_mg_graph_replay(self._h))
 
-    def solve_device(self, f_dev_ptr: int, p_dev_ptr: int, stream_ptr: int = 0) -> SolveResult:
+    def solve_device(self, f_dev_ptr: int, p_dev_ptr: int, stream_ptr: int = 0,
+                     warn: bool = True) -> SolveResult:
         """Device-resident solve on interior-packed device vectors (e.g. torch
-        tensors' data_ptr()); the fields never leave HBM."""
+        tensors' data_ptr()); the fields never leave HBM.  warn=False leaves
+        non-convergence to the returned status (no NotConverged warning)."""
         rmax = ctypes.c_double(0.0)
         iters = ctypes.c_int32(0)
         self._hist = np.zeros(self.maxcy + 1)
@@ -244,4 +257,4 @@ class PressureMultigrid:
             self._h, ctypes.c_void_p(f_dev_ptr), ctypes.c_void_p(p_dev_ptr),
             ctypes.c_void_p(stream_ptr), ctypes.byref(rmax), ctypes.byref(iters),
             self._hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-        return self._finish(rc, rmax, iters)
+        return self._finish(rc, rmax, iters, warn)

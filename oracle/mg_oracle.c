@@ -236,6 +236,30 @@ void orc_coeff(orc_mg *mg) {
                         if (bk->nbr[FN] < 0 && bk->bc[FN] != 5 && j == n[1]) an[q] = 0.0;
                         if (bk->nbr[FB] < 0 && bk->bc[FB] != 5 && k == 1) ab[q] = 0.0;
                         if (bk->nbr[FT] < 0 && bk->bc[FT] != 5 && k == n[2]) at[q] = 0.0;
+                        if (mg->lsm == 2 && l > 0) {
+                            /* face-coefficient restriction (the library's
+                             * MF3D_COARSE_FACE; not in the reference): each coarse
+                             * face = sum of the 4 fine faces it covers / 16, the
+                             * lower tangential axis fastest */
+                            const int32_t *nf = DIMS(mg, b, l - 1);
+                            const double *cf = mg->cof[SLOT(mg, b, l - 1)];
+                            const size_t NF = isz(nf);
+                            const int fi = 2 * i - 1, fj = 2 * j - 1, fk = 2 * k - 1;
+#define CF(w, a, bb, c) cf[(size_t)(w) * NF + ci(nf, a, bb, c)]
+                            aw[q] = (CF(0, fi, fj, fk) + CF(0, fi, fj + 1, fk) + CF(0, fi, fj, fk + 1) +
+                                     CF(0, fi, fj + 1, fk + 1)) / 16.0;
+                            ae[q] = (CF(1, fi + 1, fj, fk) + CF(1, fi + 1, fj + 1, fk) + CF(1, fi + 1, fj, fk + 1) +
+                                     CF(1, fi + 1, fj + 1, fk + 1)) / 16.0;
+                            as[q] = (CF(2, fi, fj, fk) + CF(2, fi + 1, fj, fk) + CF(2, fi, fj, fk + 1) +
+                                     CF(2, fi + 1, fj, fk + 1)) / 16.0;
+                            an[q] = (CF(3, fi, fj + 1, fk) + CF(3, fi + 1, fj + 1, fk) + CF(3, fi, fj + 1, fk + 1) +
+                                     CF(3, fi + 1, fj + 1, fk + 1)) / 16.0;
+                            ab[q] = (CF(4, fi, fj, fk) + CF(4, fi + 1, fj, fk) + CF(4, fi, fj + 1, fk) +
+                                     CF(4, fi + 1, fj + 1, fk)) / 16.0;
+                            at[q] = (CF(5, fi, fj, fk + 1) + CF(5, fi + 1, fj, fk + 1) + CF(5, fi, fj + 1, fk + 1) +
+                                     CF(5, fi + 1, fj + 1, fk + 1)) / 16.0;
+#undef CF
+                        }
                         /* :205-208 */
                         ap[q] = -1.0 * (aw[q] + ae[q] + as[q] + an[q] + ab[q] + at[q]);
                         rh[q] = 0.0; /* :210 */
@@ -632,5 +656,104 @@ int orc_solve(orc_mg *mg, const double *f, double *p, double tol, int maxcy, int
     }
     free(r);
     free(pp);
+    return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GCR(1) with the V-cycle as preconditioner: the library's krylov =         */
+/* MF3D_OUTER_GCR outer loop (not in the reference).  Sums run in plain      */
+/* order, so results agree with the library's tree sums to rounding only.   */
+/* ------------------------------------------------------------------------ */
+static double dot_int(orc_mg *mg, const double *a_i, const double *b_i) {
+    double s = 0.0;
+    for (size_t q = 0, n = 0; n < (size_t)mg->nb; ++n) {
+        const size_t m = isz(DIMS(mg, (int)n, 0));
+        for (size_t e = 0; e < m; ++e, ++q) s += a_i[q] * b_i[q];
+    }
+    return s;
+}
+
+/* out = A u on level-0 interiors (u ghosted with mapped ghosts) */
+static void apply_a(orc_mg *mg, const double *u_g, const double *zero, double *out) {
+    size_t og = 0, oi = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        residual_block(mg, b, 0, u_g + og, zero + oi, out + oi); /* 0 - A u */
+        const size_t m = isz(DIMS(mg, b, 0));
+        for (size_t q = 0; q < m; ++q) out[oi + q] = -out[oi + q];
+        og += gsz(DIMS(mg, b, 0));
+        oi += m;
+    }
+}
+
+int orc_solve_gcr(orc_mg *mg, const double *f, double *p, double tol, int maxcy, int min_cycles,
+                  double *hist_rho, double *hist_rmax) {
+    size_t ntot = 0, gtot = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        ntot += isz(DIMS(mg, b, 0));
+        gtot += gsz(DIMS(mg, b, 0));
+    }
+    double *r = malloc(ntot * sizeof(double)), *w = calloc(ntot, sizeof(double));
+    double *q = malloc(ntot * sizeof(double)), *zero = calloc(ntot, sizeof(double));
+    double *z = malloc(gtot * sizeof(double)), *d = calloc(gtot, sizeof(double));
+    for (int b = 0; b < mg->nb; ++b) memset(mg->pf[b], 0, gsz(DIMS(mg, b, 0)) * sizeof(double));
+    size_t off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        residual_block(mg, b, 0, mg->pf[b], f + off, r + off);
+        off += isz(DIMS(mg, b, 0));
+    }
+    double nf, rm, nr;
+    norms(mg, f, &nf, &rm);
+    norms(mg, r, &nr, &rm);
+    if (nf == 0.0) nf = 1.0;
+    hist_rho[0] = nr / nf;
+    hist_rmax[0] = rm;
+    double gamma = 1.0;
+    int k = 0;
+    for (k = 1; k <= maxcy; ++k) {
+        memset(z, 0, gtot * sizeof(double));
+        orc_vcycle(mg, r, z);
+        apply_a(mg, z, zero, q);
+        const double beta = k > 1 && gamma != 0.0 ? dot_int(mg, q, w) / gamma : 0.0;
+        for (size_t g = 0; g < gtot; ++g) d[g] = z[g] + (-beta) * d[g];
+        apply_a(mg, d, zero, w);
+        gamma = dot_int(mg, w, w);
+        const double alpha = gamma != 0.0 ? dot_int(mg, r, w) / gamma : 0.0;
+        size_t og = 0;
+        for (int b = 0; b < mg->nb; ++b) {
+            const int32_t *n = DIMS(mg, b, 0);
+            for (int kk = 1; kk <= n[2]; ++kk)
+                for (int j = 1; j <= n[1]; ++j)
+                    for (int i = 1; i <= n[0]; ++i) {
+                        size_t g = gi(n, i, j, kk);
+                        mg->pf[b][g] = mg->pf[b][g] + alpha * d[og + g];
+                    }
+            og += gsz(n);
+        }
+        exchange_fields(mg, 0, mg->pf);
+        off = 0;
+        for (int b = 0; b < mg->nb; ++b) {
+            residual_block(mg, b, 0, mg->pf[b], f + off, r + off);
+            off += isz(DIMS(mg, b, 0));
+        }
+        norms(mg, r, &nr, &rm);
+        hist_rho[k] = nr / nf;
+        hist_rmax[k] = rm;
+        if (hist_rho[k] <= tol && k >= min_cycles) break;
+    }
+    if (k > maxcy) k = maxcy;
+    off = 0;
+    for (int b = 0; b < mg->nb; ++b) {
+        const int32_t *n = DIMS(mg, b, 0);
+        for (int kk = 1; kk <= n[2]; ++kk)
+            for (int j = 1; j <= n[1]; ++j)
+                for (int i = 1; i <= n[0]; ++i) p[off + ci(n, i, j, kk)] = mg->pf[b][gi(n, i, j, kk)];
+        off += isz(n);
+    }
+    free(r);
+    free(w);
+    free(q);
+    free(zero);
+    free(z);
+    free(d);
     return k;
 }

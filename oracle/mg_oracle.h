@@ -29,6 +29,9 @@ typedef struct {
 
 typedef struct orc_mg orc_mg;
 
+/* lsm: 0 constant coefficients; 1 the reference's density hierarchy
+ * (mgsolver.for:69-181); 2 the same finest level with coarse operators by
+ * face-coefficient restriction (the library's MF3D_COARSE_FACE). */
 orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
                    int nu_pre, int nu_post);
 void    orc_destroy(orc_mg *mg);
@@ -45,6 +48,11 @@ void    orc_vcycle(orc_mg *mg, const double *rhs, double *pp);
  * Returns the number of cycles run. */
 int     orc_solve(orc_mg *mg, const double *f, double *p, double tol, int maxcy,
                   int min_cycles, double *hist_rho, double *hist_rmax);
+/* The same problem with GCR(1) around the V-cycle (the library's
+ * MF3D_OUTER_GCR): beta = (A z, w_prev)/(w_prev, w_prev), d = z - beta d_prev,
+ * w = A d, alpha = (r, w)/(w, w), p += alpha d, r = f - A p. */
+int     orc_solve_gcr(orc_mg *mg, const double *f, double *p, double tol, int maxcy,
+                      int min_cycles, double *hist_rho, double *hist_rmax);
 
 /* Unit-level access for kernel tests. */
 int     orc_dims(const orc_mg *mg, int block, int level, int32_t out_n[3]);

@@ -44,6 +44,8 @@ def lib():
         L.orc_vcycle.argtypes = [vp, dp, dp]
         L.orc_solve.restype = ip
         L.orc_solve.argtypes = [vp, dp, dp, ctypes.c_double, ip, ip, dp, dp]
+        L.orc_solve_gcr.restype = ip
+        L.orc_solve_gcr.argtypes = [vp, dp, dp, ctypes.c_double, ip, ip, dp, dp]
         L.orc_dims.argtypes = [vp, ip, ip, ctypes.POINTER(ctypes.c_int32)]
         for name in ("orc_phi", "orc_rhs", "orc_dens"):
             getattr(L, name).restype = dp
@@ -71,7 +73,11 @@ def _concat(parts) -> np.ndarray:
 class OracleMG:
     """FP64 multigrid of the reference, restated on the CPU."""
 
-    def __init__(self, blocks, levels: int, lsm: bool = False, nu_pre: int = 2, nu_post: int = 2):
+    def __init__(self, blocks, levels: int, lsm: bool = False, nu_pre: int = 2, nu_post: int = 2,
+                 coarse_op: str = "reference"):
+        """lsm: variable (density) coefficients; coarse_op "face" builds their
+        coarse operators by face-coefficient restriction instead of the
+        reference's density hierarchy."""
         self.blocks = list(blocks)
         self.levels = levels
         self.lsm = lsm
@@ -82,7 +88,8 @@ class OracleMG:
             arr[i].nbr[:] = list(b.nbr)
             arr[i].per[:] = list(b.per)
             arr[i].bc[:] = list(b.bc)
-        self._h = lib().orc_create(len(self.blocks), arr, levels, int(lsm), nu_pre, nu_post)
+        mode = (2 if coarse_op == "face" else 1) if lsm else 0
+        self._h = lib().orc_create(len(self.blocks), arr, levels, mode, nu_pre, nu_post)
         if not self._h:
             raise ValueError("oracle rejected the block set (extent not divisible by 2**(L-1)?)")
 
@@ -121,14 +128,15 @@ class OracleMG:
             o += m
         return out
 
-    def solve(self, f_blocks, tol: float, maxcy: int, min_cycles: int = 1):
-        """Poisson-form outer loop from p = 0.  Returns (p_blocks, rho, rmax, cycles)."""
+    def solve(self, f_blocks, tol: float, maxcy: int, min_cycles: int = 1, gcr: bool = False):
+        """Poisson-form outer loop from p = 0 (GCR(1) around the V-cycle with
+        gcr=True).  Returns (p_blocks, rho, rmax, cycles)."""
         f = _concat(f_blocks)
         p = np.zeros_like(f)
         rho = np.zeros(maxcy + 1)
         rmax = np.zeros(maxcy + 1)
-        k = lib().orc_solve(self._h, _dptr(f), _dptr(p), tol, maxcy, min_cycles,
-                            _dptr(rho), _dptr(rmax))
+        fn = lib().orc_solve_gcr if gcr else lib().orc_solve
+        k = fn(self._h, _dptr(f), _dptr(p), tol, maxcy, min_cycles, _dptr(rho), _dptr(rmax))
         return self._split(p), rho[:k + 1].copy(), rmax[:k + 1].copy(), k
 
     def _split(self, flat):
