@@ -104,8 +104,11 @@ def run_serving(count):
     srv = S.Server(slots=4, precision="mixed")
     out = {"slots": 4, "precision": "mixed", "requests_per_rate": count, "rates": []}
     try:
+        # warm pool: one resident hierarchy per slot and shape
+        shapes = [(n, tp) for n in S.SIZES for tp in (False, True)]
         warm = [S.Request(i, 0.0, n, 1e-6, tp, W_SEED + 5) for i, (n, tp) in
-                enumerate((n, tp) for n in S.SIZES for tp in (False, True))]
+                enumerate(sh for sh in shapes for _ in range(4))]
+        srv.pool.reserve(shapes, 4)
         srv.run(warm, realtime=False)
         for rate in (100.0, 400.0, 1600.0):
             reqs = S.poisson_requests(rate, count, W_SEED + 50)

@@ -24,6 +24,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,11 +46,12 @@ constexpr int kTileX = 64, kTileY = 4;
 struct Level {
     std::vector<BlockLevel> hb;
     BlockLevel *db = nullptr;
-    Tile *t_sw = nullptr, *t_in = nullptr, *t_row = nullptr;
-    int n_sw = 0, n_in = 0, n_row = 0;
+    Tile *t_sw = nullptr, *t_in = nullptr, *t_row = nullptr, *t_oct = nullptr;
+    int n_sw = 0, n_in = 0, n_row = 0, n_oct = 0;
     int sw_ty = 4;  // sweep tile rows
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
+    bool own_ghosts = false;  // lone block: the sweep writes the ghosts, no gather
     long long cells_gh = 0;  // arena length (all blocks, ghosted)
     void *phi[2] = {nullptr, nullptr};
     int cur = 0;  // which phi holds the current potential (host bookkeeping)
@@ -264,17 +267,32 @@ std::vector<Tile> row_tiles(const std::vector<BlockLevel> &hb) {
     return t;
 }
 
-// Sweep tiles: 64 x ty columns, a chunk of tz planes.  tz is chosen so the
-// level offers at least ~1024 workgroups where it can.
-std::vector<Tile> sweep_tiles(const std::vector<BlockLevel> &hb, int ty) {
+// Octet tiles of the prolongation: 4 coarse-index rows m_y (fine rows
+// 2m_y, 2m_y+1) of one coarse-index plane m_z, m = 0 .. n/2 on each axis.
+std::vector<Tile> octet_tiles(const std::vector<BlockLevel> &hb) {
+    std::vector<Tile> t;
+    for (int b = 0; b < (int)hb.size(); ++b) {
+        const BlockLevel &g = hb[b];
+        for (int z = 0; 2 * z <= g.n2 + 1; ++z)
+            for (int y = 0; 2 * y <= g.n1 + 1; y += kTileY) t.push_back(Tile{b, 0, y, z, z});
+    }
+    return t;
+}
+
+// Sweep tiles: 64 x ty columns, a chunk of tz planes (even, so chunks start
+// at odd planes).  tz = 0: the deepest chunk (at most 32) that still gives
+// the level ~1024 workgroups where it can.
+std::vector<Tile> sweep_tiles(const std::vector<BlockLevel> &hb, int ty, int tz = 0) {
     long long cols = 0;
     int nzmax = 1;
     for (const BlockLevel &g : hb) {
         cols += (long long)((g.n0 + SW_TX - 1) / SW_TX) * ((g.n1 + ty - 1) / ty);
         nzmax = std::max(nzmax, g.n2);
     }
-    int tz = 32;
-    while (tz > 2 && cols * ((nzmax + tz - 1) / tz) < 1024) tz >>= 1;
+    if (tz <= 0) {
+        tz = 32;
+        while (tz > 2 && cols * ((nzmax + tz - 1) / tz) < 1024) tz >>= 1;
+    }
     std::vector<Tile> t;
     for (int b = 0; b < (int)hb.size(); ++b) {
         const BlockLevel &g = hb[b];
@@ -386,7 +404,7 @@ void fill(mf3d_mg *mg, T *a, long long n, const int *done = nullptr) {
 template <typename T>
 void gather(mf3d_mg *mg, int l, const int *done) {
     Level &L = mg->lv[l];
-    if (!L.n_full) return;
+    if (!L.n_full || L.own_ghosts) return;
     const int gx = std::max(1, std::min(64, (L.max_full + 255) / 256));
     hipLaunchKernelGGL(k_gather<T>, dim3(gx, L.n_full), dim3(256), 0, mg->stream, (T *)L.phi[L.cur], L.db,
                        L.r_full, done);
@@ -400,13 +418,14 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
     const T *pin = (const T *)L.phi[L.cur];
     T *pout = (T *)L.phi[1 - L.cur];
     const uint32_t bytes = (uint32_t)(L.cells_gh * (long long)sizeof(T));
+    const int own = L.own_ghosts ? 1 : 0;
     if (L.sw_ty == 8)
         hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
-                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, L.db,
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
                            L.t_sw, done);
     else
         hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
-                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, L.db,
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
                            L.t_sw, done);
 }
 
@@ -446,9 +465,9 @@ void prolong_to(mf3d_mg *mg, int lf, const int *done) {
     Level &C = mg->lv[lf + 1], &F = mg->lv[lf];
     const bool tk = tk_on(mg, MF3D_K_PROLONG, lf);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL(k_prolong<T>, dim3(F.n_row), dim3(kTileX, kTileY), 0, mg->stream,
+    hipLaunchKernelGGL(k_prolong<T>, dim3(F.n_oct), dim3(kTileX, kTileY), 0, mg->stream,
                        (const T *)F.phi[F.cur], (T *)F.phi[1 - F.cur], F.db, (const T *)C.phi[C.cur], C.db,
-                       F.t_row, done);
+                       F.t_oct, done);
     if (tk) tk_mark(mg);
     F.cur = 1 - F.cur;
 }
@@ -789,6 +808,9 @@ int choose_tail(mf3d_mg *mg) {
     return upload(mg, &mg->d_tail, tl);
 }
 
+template <typename T, bool VAR>
+int autotune_sweeps(mf3d_mg *mg);
+
 int build(mf3d_mg *mg) {
     const mf3d_mg_cfg &c = mg->cfg;
     const int L = c.levels, nb = c.nblocks;
@@ -843,11 +865,26 @@ int build(mf3d_mg *mg) {
         if ((rc = upload(mg, &V.db, V.hb))) return rc;
         auto tsw = xcd_order(sweep_tiles(V.hb, V.sw_ty)), tin = xcd_order(cell_tiles(V.hb)),
              trow = xcd_order(row_tiles(V.hb));
+        auto toct = xcd_order(octet_tiles(V.hb));
         V.n_sw = (int)tsw.size(); V.n_in = (int)tin.size(); V.n_row = (int)trow.size();
+        V.n_oct = (int)toct.size();
+        if ((rc = upload(mg, &V.t_oct, toct))) return rc;
         if ((rc = upload(mg, &V.t_sw, tsw)) || (rc = upload(mg, &V.t_in, tin)) || (rc = upload(mg, &V.t_row, trow)))
             return rc;
         auto rf = build_regions(mg->blk, l, &V.max_full);
         V.n_full = (int)rf.size();
+        // lone blocks (no neighbour, no periodic partner): each sweep stores the
+        // ghost images itself and no gather runs.  MF3D_OWN_GHOSTS=0: gather.
+        {
+            bool lone = true;
+            for (const mf3d_block_desc &B : mg->blk) {
+                for (int f = 0; f < 6; ++f) lone = lone && B.nbr[f] < 0 && B.per[f] < 0;
+                for (int d = 0; d < 3; ++d)
+                    lone = lone && dim_at(B, d, l) >= 2 && is_neumann(B, 2 * d) == is_neumann(B, 2 * d + 1);
+            }
+            const char *env = std::getenv("MF3D_OWN_GHOSTS");
+            V.own_ghosts = lone && !(env && env[0] == '0');
+        }
         if ((rc = upload(mg, &V.r_full, rf))) return rc;
         if ((rc = dev_alloc(mg, &V.phi[0], V.cells_gh * w)) || (rc = dev_alloc(mg, &V.phi[1], V.cells_gh * w)) ||
             (rc = dev_alloc(mg, &V.rhs, V.cells_gh * w)))
@@ -911,6 +948,106 @@ int build(mf3d_mg *mg) {
     fill<double>(mg, mg->pbuf[1], L0.cells_gh);
     fill<double>(mg, mg->f_g, L0.cells_gh);
     hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st, mg->cfg.eps);
+    HIPCHK(hipStreamSynchronize(mg->stream));
+    HIPCHK(hipGetLastError());
+    if ((rc = mixed ? (c.variable_coef ? autotune_sweeps<float, true>(mg) : autotune_sweeps<float, false>(mg))
+                    : (c.variable_coef ? autotune_sweeps<double, true>(mg) : autotune_sweeps<double, false>(mg))))
+        return rc;
+    return 0;
+}
+
+// Tile autotuner: for every level the sweep kernel runs on, time the sweep
+// over chunk depths tz = 2 .. 64 (the finest level with 8-row tiles, the
+// others with 4-row tiles) and keep the fastest.  Each candidate is launched
+// once to warm up and then timed over 3 launches with HIP events on the
+// handle's stream, on the level's own (zeroed) arrays; the fields are zeroed
+// again afterwards.  The tiling never changes a result.  MF3D_AUTOTUNE=0
+// keeps the default depths.
+// Chosen depths are remembered per process, keyed by precision, coefficient
+// form, tile rows and the level's block extents, so later handles of the
+// same shape (the serving pool's) skip the timing.
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune_tz;
+
+template <typename T, bool VAR>
+int autotune_sweeps(mf3d_mg *mg) {
+    const char *env = std::getenv("MF3D_AUTOTUNE");
+    if (env && env[0] == '0') return 0;
+    auto key_of = [&](const Level &V) {
+        std::string k = std::to_string(sizeof(T)) + (VAR ? "v" : "c") + std::to_string(V.sw_ty);
+        for (const BlockLevel &g : V.hb)
+            k += ":" + std::to_string(g.n0) + "," + std::to_string(g.n1) + "," + std::to_string(g.n2);
+        return k;
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = 0;
+    for (int l = 0; l < std::min(mg->lt, (int)mg->lv.size()) && !rc; ++l) {
+        Level &V = mg->lv[l];
+        const std::string key = key_of(V);
+        int cached = 0;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_tz.find(key);
+            if (it != g_tune_tz.end()) cached = it->second;
+        }
+        if (cached) {
+            auto t = xcd_order(sweep_tiles(V.hb, V.sw_ty, cached));
+            Tile *dst = nullptr;
+            if ((rc = upload(mg, &dst, t))) break;
+            V.t_sw = dst;
+            V.n_sw = (int)t.size();
+            continue;
+        }
+        int nzmax = 1;
+        for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
+        float best = 1e30f;
+        int best_tz = 0;
+        std::vector<Tile> best_t;
+        Tile *tmp = nullptr;
+        for (int tz = 2; tz <= 64 && tz <= std::max(2, nzmax); tz *= 2) {
+            auto t = xcd_order(sweep_tiles(V.hb, V.sw_ty, tz));
+            if (hipMalloc((void **)&tmp, sizeof(Tile) * t.size()) != hipSuccess) break;
+            Tile *keep = V.t_sw;
+            const int keep_n = V.n_sw;
+            V.t_sw = tmp;
+            V.n_sw = (int)t.size();
+            float ms = 0.f;
+            if (hipMemcpyAsync(tmp, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, mg->stream) ==
+                hipSuccess) {
+                sweep_launch<T, VAR, false>(mg, V, nullptr);
+                (void)hipEventRecord(e0, mg->stream);
+                for (int r = 0; r < 3; ++r) sweep_launch<T, VAR, false>(mg, V, nullptr);
+                (void)hipEventRecord(e1, mg->stream);
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    ms = 1e30f;
+            }
+            V.t_sw = keep;
+            V.n_sw = keep_n;
+            (void)hipFree(tmp);
+            if (ms < best) {
+                best = ms;
+                best_tz = tz;
+                best_t = std::move(t);
+            }
+        }
+        if (!best_t.empty()) {
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tune_tz[key] = best_tz;
+            }
+            Tile *dst = nullptr;
+            if ((rc = upload(mg, &dst, best_t))) break;
+            V.t_sw = dst;
+            V.n_sw = (int)best_t.size();
+        }
+        fill<T>(mg, (T *)V.phi[0], V.cells_gh);
+        fill<T>(mg, (T *)V.phi[1], V.cells_gh);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
     return 0;

@@ -47,12 +47,26 @@ constexpr int SW_UNROLL = 10;
 template <int J>
 using Step = std::integral_constant<int, J>;
 
+// Ghost images of a lone block (own_ghosts: no neighbour, no periodic
+// partner, the same kind of face at both ends of an axis, every extent at
+// least 2).  On each axis the ghost at 0 holds the interior cell m0 (1 on a
+// Neumann face, n on a periodic one) and the ghost at n+1 holds mN (n or
+// 1): the values mgbound + exchangepp + exchbc_mgpp leave there after a
+// sweep (mgsolver.for:610-611), edges and corners taking the image on every
+// axis.  So a cell has at most one image coordinate per axis; the thread
+// that stores it also stores the copies in the (up to 7) ghosts that map
+// to it, and no gather runs.
+__device__ __forceinline__ int ghost_image(int v, int n, bool neu_lo, bool neu_hi) {
+    const int m0 = neu_lo ? 1 : n, mn = neu_hi ? n : 1;
+    return v == m0 ? 0 : (v == mn ? n + 1 : -1);
+}
+
 template <typename T, bool VAR, bool ZERO, bool RCP, int TY>
 __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T *__restrict__ pout,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ cx,
                                                   const T *__restrict__ cy,
-                                                  const T *__restrict__ cz, uint32_t bytes,
+                                                  const T *__restrict__ cz, uint32_t bytes, int own_ghosts,
                                                   const BlockLevel *__restrict__ bls,
                                                   const Tile *__restrict__ tiles,
                                                   const int *__restrict__ done) {
@@ -117,6 +131,12 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
         bo[par] = ob + (uint32_t)(bok[par] ? x + sy * by : 0) * W;
         bx[par] = x;
     }
+    // ghost images of this thread's stored cells (own_ghosts), per parity
+    int bix[2];
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+        bix[par] = own_ghosts && bok[par] ? ghost_image(bx[par], g.n0, g.neu & kNeuW, g.neu & kNeuE) : -1;
+    const int biy = own_ghosts && by <= g.n1 ? ghost_image(by, g.n1, g.neu & kNeuS, g.neu & kNeuN) : -1;
 
     // hoisted constant coefficients where no Neumann zero is in reach:
     // planes zf0..zf1 of this tile
@@ -232,6 +252,22 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
             else
                 v = P[l];
             bstore<T>(rs_out, v, bo[1 - PR], (uint32_t)z * pb);
+            const int iz = own_ghosts ? ghost_image(z, g.n2, g.neu & kNeuB, g.neu & kNeuT) : -1;
+            const int ix = bix[1 - PR], X = bx[1 - PR];
+            if (ix >= 0 || biy >= 0 || iz >= 0) {
+                auto img = [&](int xx, int yy, int zz) {
+                    bstore<T>(rs_out, v, ob + (uint32_t)(xx + sy * yy) * W, (uint32_t)zz * pb);
+                };
+                if (ix >= 0) img(ix, by, z);
+                if (biy >= 0) img(X, biy, z);
+                if (ix >= 0 && biy >= 0) img(ix, biy, z);
+                if (iz >= 0) {
+                    img(X, by, iz);
+                    if (ix >= 0) img(ix, by, iz);
+                    if (biy >= 0) img(X, biy, iz);
+                    if (ix >= 0 && biy >= 0) img(ix, biy, iz);
+                }
+            }
         }
     };
 

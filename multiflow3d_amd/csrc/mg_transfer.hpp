@@ -56,7 +56,12 @@ __global__ __launch_bounds__(256) void k_restrict(
 // reflected source is never one this launch has already updated.  The eight
 // coarse terms are summed in the reference's order c, c+j, c+k, c+jk, c+i,
 // c+ij, c+ik, c+ijk, weight 3 per axis on the child's side and 1 otherwise.
-// Row tiles: block b, 4 rows from y0, plane z0; threads stride along x.
+//
+// A thread owns the fine octet (2m .. 2m+1) on every axis (ghosted fine
+// indices 0 .. n+1, m = 0 .. n/2): all eight read the same eight coarse
+// values (coarse index m on each axis; a reflected ghost maps to the same
+// m), loaded once.  Tiles: block b, coarse rows m_y = y0 .. y0+3 of the
+// coarse plane m_z = z0; threads stride along m_x.
 template <typename T>
 __global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ fin, T *__restrict__ fout,
                                                  const BlockLevel *__restrict__ blf,
@@ -67,33 +72,53 @@ __global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ fin, T *_
     if (done && *done) return;
     const Tile t = tiles[blockIdx.x];
     const BlockLevel gf = blf[t.b];
-    const int j = t.y0 + threadIdx.y, k = t.z0;
-    if (j > gf.n1 + 1) return;
+    const int my = t.y0 + threadIdx.y, mz = t.z0;
+    if (2 * my > gf.n1 + 1) return;
     const BlockLevel gc = blc[t.b];
-    int sj = j, sk = k;
-    if (j == 0 && (gf.neu & kNeuS)) sj = 1;
-    if (j == gf.n1 + 1 && (gf.neu & kNeuN)) sj = gf.n1;
-    if (k == 0 && (gf.neu & kNeuB)) sk = 1;
-    if (k == gf.n2 + 1 && (gf.neu & kNeuT)) sk = gf.n2;
-    const int jc = sj >> 1, kc = sk >> 1, fj = sj & 1, fk = sk & 1;
-    const T wj0 = fj ? T(1) : T(3), wj1 = fj ? T(3) : T(1);
-    const T wk0 = fk ? T(1) : T(3), wk1 = fk ? T(3) : T(1);
-    for (int i = threadIdx.x; i <= gf.n0 + 1; i += blockDim.x) {
-        int si = i;
-        if (i == 0 && (gf.neu & kNeuW)) si = 1;
-        if (i == gf.n0 + 1 && (gf.neu & kNeuE)) si = gf.n0;
-        const int ic = si >> 1, fi = si & 1;
-        const T wi0 = fi ? T(1) : T(3), wi1 = fi ? T(3) : T(1);
-        const long long c = gidx(gc, ic, jc, kc);
-        T s = (wi0 * wj0 * wk0) * phic[c];
-        s = s + (wi0 * wj1 * wk0) * phic[c + gc.sy];
-        s = s + (wi0 * wj0 * wk1) * phic[c + gc.sz];
-        s = s + (wi0 * wj1 * wk1) * phic[c + gc.sy + gc.sz];
-        s = s + (wi1 * wj0 * wk0) * phic[c + 1];
-        s = s + (wi1 * wj1 * wk0) * phic[c + 1 + gc.sy];
-        s = s + (wi1 * wj0 * wk1) * phic[c + 1 + gc.sz];
-        s = s + (wi1 * wj1 * wk1) * phic[c + 1 + gc.sy + gc.sz];
-        fout[gidx(gf, i, j, k)] = fin[gidx(gf, si, sj, sk)] + s / T(64);
+    // fine rows / planes of the octet, their reflected sources and weights
+    int sj[2], sk[2];
+    T wj0[2], wj1[2], wk0[2], wk1[2];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        const int j = 2 * my + d, k = 2 * mz + d;
+        int a = j, b = k;
+        if (j == 0 && (gf.neu & kNeuS)) a = 1;
+        if (j == gf.n1 + 1 && (gf.neu & kNeuN)) a = gf.n1;
+        if (k == 0 && (gf.neu & kNeuB)) b = 1;
+        if (k == gf.n2 + 1 && (gf.neu & kNeuT)) b = gf.n2;
+        sj[d] = a;
+        sk[d] = b;
+        wj0[d] = (a & 1) ? T(1) : T(3);
+        wj1[d] = (a & 1) ? T(3) : T(1);
+        wk0[d] = (b & 1) ? T(1) : T(3);
+        wk1[d] = (b & 1) ? T(3) : T(1);
+    }
+    for (int mx = threadIdx.x; 2 * mx <= gf.n0 + 1; mx += blockDim.x) {
+        const long long c = gidx(gc, mx, my, mz);
+        const T c000 = phic[c], c010 = phic[c + gc.sy], c001 = phic[c + gc.sz], c011 = phic[c + gc.sy + gc.sz];
+        const T c100 = phic[c + 1], c110 = phic[c + 1 + gc.sy], c101 = phic[c + 1 + gc.sz],
+                c111 = phic[c + 1 + gc.sy + gc.sz];
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int i = 2 * mx + dx;
+                    int si = i;
+                    if (i == 0 && (gf.neu & kNeuW)) si = 1;
+                    if (i == gf.n0 + 1 && (gf.neu & kNeuE)) si = gf.n0;
+                    const T wi0 = (si & 1) ? T(1) : T(3), wi1 = (si & 1) ? T(3) : T(1);
+                    T s = (wi0 * wj0[dy] * wk0[dz]) * c000;
+                    s = s + (wi0 * wj1[dy] * wk0[dz]) * c010;
+                    s = s + (wi0 * wj0[dy] * wk1[dz]) * c001;
+                    s = s + (wi0 * wj1[dy] * wk1[dz]) * c011;
+                    s = s + (wi1 * wj0[dy] * wk0[dz]) * c100;
+                    s = s + (wi1 * wj1[dy] * wk0[dz]) * c110;
+                    s = s + (wi1 * wj0[dy] * wk1[dz]) * c101;
+                    s = s + (wi1 * wj1[dy] * wk1[dz]) * c111;
+                    fout[gidx(gf, i, 2 * my + dy, 2 * mz + dz)] = fin[gidx(gf, si, sj[dy], sk[dz])] + s / T(64);
+                }
     }
 }
 

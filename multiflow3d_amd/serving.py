@@ -171,6 +171,15 @@ class SolverPool:
         self.created += 1
         return mg
 
+    def reserve(self, shapes, per_shape: int):
+        """Create `per_shape` idle handles of each (n, two_phase) shape up front
+        (within the budget), so a burst of same-shape requests finds them."""
+        for n, tp in shapes:
+            made = [self.acquire(n, tp) for _ in range(per_shape - len(self.idle[(n, tp)]))]
+            for mg in made:
+                if mg is not None:
+                    self.release(n, tp, mg)
+
     def release(self, n: int, two_phase: bool, mg: PressureMultigrid):
         self.idle[(n, two_phase)].append(mg)
         self._lru[id(mg)] = ((n, two_phase), mg)
