@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc_mg.so")
 REF_BIN = os.path.join(HERE, "_ref", "mg_ref")
+REF_VEL_BIN = os.path.join(HERE, "_ref", "mg_ref_vel")
 
 
 class OrcBlock(ctypes.Structure):
@@ -241,3 +242,50 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
         m = (nx + 2) * (ny + 2) * (nz + 2)
         pp1.append(raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy()); o += m
     return p, rho[:k + 1].copy(), rmax[:k + 1].copy(), k, pp1
+
+
+def have_ref_vel() -> bool:
+    return os.path.exists(REF_VEL_BIN)
+
+
+def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: int, dt: float,
+                alfapr: float, lsm: bool = False, dens=None, nu_pre: int = 2, nu_post: int = 2):
+    """Run the compiled reference's velocity-form newsolv_mg loop (calvel,
+    calmas, mgkcyc, p update, exchange(4), pbound).  Every field is a list of
+    full ghosted block arrays (one ghost layer), shaped (z+2, y+2, x+2).
+    Returns (iterations, rmax history, p, u, v, w, su) with the fields as
+    lists of ghosted block arrays."""
+    if not have_ref_vel():
+        raise RuntimeError("compiled reference not built (make -C oracle ref)")
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        parts = []
+        for b in range(len(grid.blocks)):
+            parts += [ustar[b], vstar[b], wstar[b], p0[b]] + ([dens[b]] if lsm else [])
+        _concat(parts).tofile(fin)
+        fout = os.path.join(td, "out.bin")
+        ctl = os.path.join(td, "ctl.txt")
+        with open(ctl, "w") as fh:
+            nbx, nby, nbz = grid.shape
+            fh.write(f"{nbx} {nby} {nbz} {levels} {int(lsm)} {maxcy} {nu_pre} {nu_post}\n")
+            fh.write(f"{eps!r}\n")
+            fh.write(" ".join(str(c) for c in grid.bc) + "\n")
+            fh.write(" ".join(map(str, grid.ex)) + "\n")
+            fh.write(" ".join(map(str, grid.ey)) + "\n")
+            fh.write(" ".join(map(str, grid.ez)) + "\n")
+            fh.write(" ".join(repr(float(x)) for x in grid.h) + "\n")
+            fh.write(f"{dt!r} {alfapr!r}\n")
+            fh.write(fin + "\n" + fout + "\n")
+        subprocess.run([REF_VEL_BIN, ctl], check=True, capture_output=True, timeout=600)
+        raw = np.fromfile(fout, dtype=np.float64)
+    k = int(raw[0])
+    o = 1
+    rmh = raw[o:o + maxcy + 1].copy(); o += maxcy + 1
+    out = {"p": [], "u": [], "v": [], "w": [], "su": []}
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        m = (nx + 2) * (ny + 2) * (nz + 2)
+        for key in ("p", "u", "v", "w", "su"):
+            out[key].append(raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy())
+            o += m
+    return k, rmh[:k + 1], out["p"], out["u"], out["v"], out["w"], out["su"]
