@@ -153,3 +153,34 @@ def sphere_density(grid: BlockGrid, radius: float = 0.2, rho_l: float = 1000.0,
                   + (x[None, None, :] - cx) ** 2) - radius
     eps = eps_cells * max(dx, dy, dz)
     return rho_g + (rho_l - rho_g) * heaviside(phi, eps)
+
+
+def velocity_inputs(grid: BlockGrid, seed: int, wall_faces_zero: bool = True):
+    """Predicted face velocities u*, v*, w* for the velocity form of
+    newsolv_mg: per block a ghosted (z+2, y+2, x+2) array per component
+    (index i holds the face east of cell i, index 0 the face below the
+    first cell), drawn block by block, u* then v* then w*, i fastest, over
+    the whole ghosted array.  Faces on a physical non-periodic wall are set
+    to zero (no flow through walls) unless wall_faces_zero is False."""
+    out = ([], [], [])
+    pos = 0
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        shp = (nz + 2, ny + 2, nx + 2)
+        cnt = shp[0] * shp[1] * shp[2]
+        for c in range(3):
+            a = uniform_pm1(seed, pos, cnt).reshape(shp)
+            pos += cnt
+            if wall_faces_zero:
+                lo, hi = 2 * c, 2 * c + 1
+                n = blk.n[c]
+                if blk.nbr[lo] < 0 and blk.bc[lo] != PERIODIC:
+                    _axis_plane(a, c, 0)[...] = 0.0
+                if blk.nbr[hi] < 0 and blk.bc[hi] != PERIODIC:
+                    _axis_plane(a, c, n)[...] = 0.0
+            out[c].append(a)
+    return out
+
+
+def _axis_plane(a, axis, idx):
+    return a[:, :, idx] if axis == 0 else (a[:, idx, :] if axis == 1 else a[idx, :, :])
