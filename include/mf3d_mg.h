@@ -4,7 +4,11 @@
  *
  *   subroutine coeff       mgsolver.for:2-222       -> mf3d_mg_create / mf3d_mg_set_density
  *   subroutine mgkcyc      mgsolver.for:224-369     -> mf3d_mg_vcycle
- *   subroutine newsolv_mg  newsolv_mg.for:2-102     -> mf3d_mg_solve (Poisson form)
+ *   subroutine newsolv_mg  newsolv_mg.for:2-102     -> mf3d_mg_newsolv (the reference's own
+ *                                                       velocity form, with calvel, calmas,
+ *                                                       pbound and exchange(1..4),
+ *                                                       press.for:2-200, exchange.for:2-2499)
+ *                                                     and mf3d_mg_solve (Poisson form)
  *
  * The reference routines take no arguments: their inputs are module state
  * (module_vars.for:11-15, module_multidata.for:15-120).  Here the same state
@@ -32,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MF3D_MG_ABI_VERSION 2
+#define MF3D_MG_ABI_VERSION 3
 
 enum {
     MF3D_OK = 0,
@@ -122,6 +126,53 @@ int mf3d_mg_solve(mf3d_mg *mg, const double *const *f_blocks, double *const *p_b
  * when the solve is complete.  No host transfer of fields. */
 int mf3d_mg_solve_device(mf3d_mg *mg, const double *f_dev, double *p_dev, void *stream,
                          double *rmax_out, int32_t *iters_out, double *hist);
+
+/* newsolv_mg (newsolv_mg.for:2-102) in the reference's velocity form, bitwise
+ * the reference in FP64:
+ *
+ *   calvel; calmas                      (press.for:139-200, 2-75)
+ *   do iter = 1, maxcy
+ *     mgkcyc on su (sup under LSM)      (mgsolver.for:224-369)
+ *     p += pp over the ghost layer      (newsolv_mg.for:38-51)
+ *     exchange(4); pbound               (exchange.for:2-341, press.for:77-137)
+ *     calvel (u = u* - dt*alfapr*grad p [/rho_face]; exchange(1..3)); calmas
+ *     exit when rmax < eps and iter > 1 (newsolv_mg.for:55)
+ *
+ * ustar, vstar, wstar (dom(ib)%ustar ...) are read; u, v, w and p are read
+ * and written in place: calvel writes the faces isu..ieu (a face on a
+ * physical non-periodic wall keeps the caller's value) and the exchanges
+ * and pbound write the one-cell ghost layer exactly as the reference's
+ * copy sequence does (ghost layers beyond the first, which the reference
+ * refreshes when pl_ex > 0, are not touched).  Caller arrays need
+ * ghost >= 1; a face array's index i is the face east of (north of,
+ * above) cell i.  eps is the reference's absolute criterion on
+ * rmax = max|div u|*dx*dy*dz (the eps of control.cin), not mf3d_mg_cfg.eps.
+ * Under LSM (variable_coef) mf3d_mg_set_density must have received the
+ * cell density with its ghost layer; newsolv_mg.for:12-14 rebuilds the
+ * coefficients every call, so call it before each solve whose density
+ * changed.  su (may be NULL) gets the last calmas right-hand side su (sup
+ * under LSM) over the interior.  rmax_hist (maxcy+1 doubles, may be NULL)
+ * gets rmax before the first cycle and after each cycle.  Return codes as
+ * mf3d_mg_solve (rmax_out = the last rmax).  The mixed precision runs
+ * the V-cycle in FP32; p, u, v, w and su stay FP64. */
+int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const *vstar,
+                    const double *const *wstar, double *const *u, double *const *v, double *const *w,
+                    double *const *p, double dt, double alfapr, double eps, double *const *su,
+                    double *rmax_out, int32_t *iters_out, double *rmax_hist);
+
+/* The same with device-resident fields: device pointers to every block's
+ * ghosted array ((n0+2)(n1+2)(n2+2) values, i fastest) concatenated in
+ * block order, mf3d_mg_ghosted_cells values each (su likewise, interior
+ * written; may be NULL).  Runs on `stream` (NULL = the handle's own) and
+ * returns when the solve is complete.  The captured outer step is kept
+ * for the last set of field pointers. */
+int mf3d_mg_newsolv_device(mf3d_mg *mg, const double *ustar, const double *vstar, const double *wstar,
+                           double *u, double *v, double *w, double *p, double dt, double alfapr,
+                           double eps, double *su, void *stream, double *rmax_out, int32_t *iters_out,
+                           double *rmax_hist);
+
+/* Length of the ghosted device fields of mf3d_mg_newsolv_device. */
+int64_t mf3d_mg_ghosted_cells(const mf3d_mg *mg);
 
 /* Interior cells of all blocks (length of the device-resident vectors). */
 int64_t mf3d_mg_cells(const mf3d_mg *mg);

@@ -21,7 +21,8 @@ EXPORTS = (
     "mf3d_mg_solve_device", "mf3d_mg_cells", "mf3d_mg_device_bytes",
     "mf3d_mg_estimate_bytes", "mf3d_mg_last_error", "mf3d_mg_destroy",
     "mf3d_mg_abi_version", "mf3d_mg_time_kernel", "mf3d_mg_kernel_time",
-    "mf3d_mg_graph_replay", "mf3d_mg_set_eps",
+    "mf3d_mg_graph_replay", "mf3d_mg_set_eps", "mf3d_mg_newsolv", "mf3d_mg_newsolv_device",
+    "mf3d_mg_ghosted_cells",
 )
 MF3D_COARSE_REFERENCE, MF3D_COARSE_FACE = 0, 1
 MF3D_OUTER_DEFECT, MF3D_OUTER_GCR = 0, 1
@@ -93,5 +94,14 @@ def lib():
     L.mf3d_mg_graph_replay.argtypes = [vp]
     L.mf3d_mg_set_eps.restype = ctypes.c_int
     L.mf3d_mg_set_eps.argtypes = [vp, ctypes.c_double]
+    d = ctypes.c_double
+    L.mf3d_mg_newsolv.restype = ctypes.c_int
+    L.mf3d_mg_newsolv.argtypes = [vp, dpp, dpp, dpp, dpp, dpp, dpp, dpp, d, d, d, dpp, dp,
+                                  ctypes.POINTER(ctypes.c_int32), dp]
+    L.mf3d_mg_newsolv_device.restype = ctypes.c_int
+    L.mf3d_mg_newsolv_device.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, d, d, d, vp, vp, dp,
+                                         ctypes.POINTER(ctypes.c_int32), dp]
+    L.mf3d_mg_ghosted_cells.restype = ctypes.c_int64
+    L.mf3d_mg_ghosted_cells.argtypes = [vp]
     _lib = L
     return L

@@ -36,6 +36,7 @@
 #include "mg_sweep.hpp"
 #include "mg_tail.hpp"
 #include "mg_transfer.hpp"
+#include "mg_velocity.hpp"
 
 using namespace mf3d;
 
@@ -114,6 +115,22 @@ struct mf3d_mg {
     int last_replay = 0;    // the last solve replayed captured cycles
     long long tk_launches = 0;
     double tk_ms = 0.0;
+    // velocity form of newsolv_mg, built at its first call: per-block face
+    // ranges and low-face sources, the composed ghost maps of exchange(1..3)
+    // (u, v, w entries, cumulative counts vmap_c) and of exchange(4) +
+    // pbound (vmap_p entries after them), their staging buffer, the arenas
+    // of the host-array call, and the captured outer step with the field
+    // pointers it was recorded with
+    bool dens_ghosted = false;  // set_density received the density's ghost layer
+    bool vel_ready = false;
+    VelBlock *d_vb = nullptr;
+    uint2 *d_vmap = nullptr;
+    long long vmap_c[3] = {0, 0, 0};
+    long long vmap_p = 0;
+    double *d_vstage = nullptr;
+    double *vf[8] = {};  // u*, v*, w*, u, v, w, p, su
+    hipGraphExec_t vgraph = nullptr;
+    const void *vkey[7] = {};
 };
 
 namespace {
@@ -1119,6 +1136,448 @@ void drop_graphs(mf3d_mg *mg) {
         }
     mg->graphs_ready = false;
     drop_timed_graphs(mg);
+    if (mg->vgraph) {
+        (void)hipGraphExecDestroy(mg->vgraph);
+        mg->vgraph = nullptr;
+    }
+}
+
+// ----------------------------------------------------- velocity form (host)
+//
+// exchange(op) and pbound only copy values, so one call of them is a fixed
+// map: every ghost entry it writes ends up holding the value some entry
+// (interior or ghost) held before the call.  The map is found once, by
+// running the reference's copy sequence on origins instead of values, and
+// then applied on the device as a gather.  The sequence, for one ghost
+// layer on one rank with equal resolution:
+//   exchange_bc (exchange_bc.for:14-841), when any face is periodic: for
+//     every periodic face without a neighbour, in block order and face
+//     order W, E, S, N, B, T, a block alone along that axis copies its own
+//     opposite plane, otherwise it hands its boundary plane to the
+//     periodic partner; whole ghosted planes; the hand-over loop runs
+//     twice (:16-330, :332-640) and the partners unpack last (:642-838);
+//   exchange_smlvl (exchange.for:342-2499): every block packs before any
+//     block unpacks; each block then unpacks its 8 corners, its 12 edges
+//     and its 6 faces, in that order (:2118-2484);
+//   pbound (press.for:77-137, pressure only): each physical non-periodic
+//     face, W, E, S, N, B, T, copies the adjacent interior plane over its
+//     whole ghosted plane.
+
+// Ghost entries of one field, per block: the arena offset of the entry
+// whose original value each ghost holds.  Slots: the x ghost planes
+// (whole), then the y ghost planes (interior i), then the z ghost planes
+// (interior i, j).
+struct GhostOrigins {
+    const std::vector<BlockLevel> &hb;
+    std::vector<std::vector<uint64_t>> o;
+
+    explicit GhostOrigins(const std::vector<BlockLevel> &h) : hb(h), o(h.size()) {
+        for (size_t b = 0; b < h.size(); ++b) {
+            const BlockLevel &g = h[b];
+            o[b].resize(2ull * (g.n1 + 2) * (g.n2 + 2) + 2ull * g.n0 * (g.n2 + 2) + 2ull * g.n0 * g.n1);
+            for_ghosts((int)b, [&](int i, int j, int k) { o[b][slot((int)b, i, j, k)] = self(g, i, j, k); });
+        }
+    }
+    static uint64_t self(const BlockLevel &g, int i, int j, int k) {
+        return (uint64_t)(g.off + i + g.sy * j + g.sz * k);
+    }
+    long long slot(int b, int i, int j, int k) const {
+        const BlockLevel &g = hb[b];
+        const long long X = (long long)(g.n1 + 2) * (g.n2 + 2), Y = (long long)g.n0 * (g.n2 + 2);
+        if (i == 0 || i == g.n0 + 1) return (i == 0 ? 0 : X) + j + (long long)(g.n1 + 2) * k;
+        if (j == 0 || j == g.n1 + 1) return 2 * X + (j == 0 ? 0 : Y) + (i - 1) + (long long)g.n0 * k;
+        if (k == 0 || k == g.n2 + 1)
+            return 2 * X + 2 * Y + (k == 0 ? 0 : (long long)g.n0 * g.n1) + (i - 1) + (long long)g.n0 * (j - 1);
+        return -1;
+    }
+    uint64_t get(int b, int i, int j, int k) const {
+        const long long s = slot(b, i, j, k);
+        return s < 0 ? self(hb[b], i, j, k) : o[b][s];
+    }
+    void set(int b, int i, int j, int k, uint64_t v) { o[b][slot(b, i, j, k)] = v; }
+    template <typename Fn>
+    void for_ghosts(int b, Fn fn) const {
+        const BlockLevel &g = hb[b];
+        for (int k = 0; k <= g.n2 + 1; ++k)
+            for (int j = 0; j <= g.n1 + 1; ++j) {
+                const bool edge = k == 0 || k == g.n2 + 1 || j == 0 || j == g.n1 + 1;
+                if (edge)
+                    for (int i = 0; i <= g.n0 + 1; ++i) fn(i, j, k);
+                else {
+                    fn(0, j, k);
+                    fn(g.n0 + 1, j, k);
+                }
+            }
+    }
+};
+
+inline int ext_of(const BlockLevel &g, int a) { return a == 0 ? g.n0 : (a == 1 ? g.n1 : g.n2); }
+
+// (i, j, k) with coordinate `idx` on axis a and (p, q) on the two other
+// axes in increasing order.
+inline void plane_ijk(int a, int idx, int p, int q, int *ijk) {
+    ijk[a] = idx;
+    ijk[a == 0 ? 1 : 0] = p;
+    ijk[a == 2 ? 1 : 2] = q;
+}
+
+// Last face index of component comp on axis a (n, or n-1 before a physical
+// non-periodic wall without a neighbour); cell fields (comp < 0): n.
+inline int face_end(const mf3d_block_desc &B, int a, int comp) {
+    const int n = B.n[a];
+    return (comp == a && B.nbr[2 * a + 1] < 0 && B.bc[2 * a + 1] != 5) ? n - 1 : n;
+}
+
+void sym_exchange_bc(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S, int comp) {
+    std::map<std::pair<int, int>, std::vector<uint64_t>> bufs;
+    const int nb = (int)blk.size();
+    auto plane_each = [&](int b, int a, auto fn) {
+        const BlockLevel &g = S.hb[b];
+        const int e1 = ext_of(g, a == 0 ? 1 : 0) + 2, e2 = ext_of(g, a == 2 ? 1 : 2) + 2;
+        for (int q = 0; q < e2; ++q)
+            for (int p = 0; p < e1; ++p) fn(p, q);
+    };
+    for (int pass = 0; pass < 2; ++pass)
+        for (int b = 0; b < nb; ++b)
+            for (int a = 0; a < 3; ++a)
+                for (int side = 0; side < 2; ++side) {
+                    const int f = 2 * a + side;
+                    if (!(blk[b].bc[f] == 5 && blk[b].nbr[f] < 0)) continue;
+                    const int ie = face_end(blk[b], a, comp);
+                    if (blk[b].nbr[2 * a + 1 - side] < 0) {  // alone along a: own opposite plane
+                        const int dst = side == 0 ? 0 : ie + 1, src = side == 0 ? ie : 1;
+                        plane_each(b, a, [&](int p, int q) {
+                            int d[3], s[3];
+                            plane_ijk(a, dst, p, q, d);
+                            plane_ijk(a, src, p, q, s);
+                            S.set(b, d[0], d[1], d[2], S.get(b, s[0], s[1], s[2]));
+                        });
+                    } else {
+                        std::vector<uint64_t> &buf = bufs[{blk[b].per[f], 2 * a + 1 - side}];
+                        buf.clear();
+                        const int src = side == 0 ? 1 : ie;
+                        plane_each(b, a, [&](int p, int q) {
+                            int s[3];
+                            plane_ijk(a, src, p, q, s);
+                            buf.push_back(S.get(b, s[0], s[1], s[2]));
+                        });
+                    }
+                }
+    for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < 3; ++a)
+            for (int side = 0; side < 2; ++side) {
+                const int f = 2 * a + side;
+                if (!(blk[b].bc[f] == 5 && blk[b].nbr[f] < 0 && blk[b].nbr[2 * a + 1 - side] >= 0)) continue;
+                const int dst = side == 0 ? 0 : face_end(blk[b], a, comp) + 1;
+                const auto it = bufs.find({b, f});
+                if (it == bufs.end()) continue;  // no partner handed a plane over
+                const std::vector<uint64_t> &buf = it->second;
+                size_t t = 0;
+                plane_each(b, a, [&](int p, int q) {
+                    int d[3];
+                    plane_ijk(a, dst, p, q, d);
+                    S.set(b, d[0], d[1], d[2], buf[t++]);
+                });
+            }
+}
+
+// Block in direction t (components -1, 0, +1) through chained face
+// neighbours (no periodic wrap), or -1: the edge and corner neighbours that
+// alloc_dom.for:617-1163 finds by matching block coordinates.  Every order
+// of the chain must agree (a conforming block grid).
+int diag_block(const std::vector<mf3d_block_desc> &blk, int b, const int *t, bool *ambiguous) {
+    int res = -2;
+    const int orders[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (const auto &ord : orders) {
+        int c = b;
+        for (int m = 0; m < 3 && c >= 0; ++m) {
+            const int a = ord[m];
+            if (t[a]) c = blk[c].nbr[2 * a + (t[a] > 0 ? 1 : 0)];
+        }
+        if (res == -2) res = c;
+        else if (res != c) *ambiguous = true;
+    }
+    return res;
+}
+
+// corprev1..4, cornext1..4, edgprev1..6, edgnext1..6 (exchange.for:2118-2389)
+constexpr int kDiagDirs[20][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1}, {1, 1, 1},
+                                  {1, -1, 1},   {-1, -1, 1}, {-1, 1, 1}, {-1, 0, -1}, {0, 1, -1},
+                                  {1, 0, -1},   {0, -1, -1}, {-1, -1, 0}, {-1, 1, 0}, {1, 0, 1},
+                                  {0, -1, 1},   {-1, 0, 1},  {0, 1, 1},  {1, 1, 0},   {1, -1, 0}};
+
+bool sym_exchange_smlvl(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
+    const GhostOrigins pre = S;
+    bool ambiguous = false;
+    const int nb = (int)blk.size();
+    for (int d = 0; d < nb; ++d) {
+        const BlockLevel &gd = S.hb[d];
+        const int nd[3] = {gd.n0, gd.n1, gd.n2};
+        // corners and edges: along an axis where the sender lies below the
+        // receiver's index 1-q takes the sender's n+1-q, above its n+q the
+        // sender's q (q = 0, 1); written where some q = 1; edges run over
+        // the interior of their free axis
+        for (const auto &t : kDiagDirs) {
+            const int s = diag_block(blk, d, t, &ambiguous);
+            if (s < 0) continue;
+            const BlockLevel &gs = S.hb[s];
+            const int ns[3] = {gs.n0, gs.n1, gs.n2};
+            bool conform = true;
+            for (int a = 0; a < 3; ++a) conform = conform && (t[a] != 0 || ns[a] == nd[a]);
+            if (!conform) {  // an edge neighbour must span the same cells along its free axis
+                ambiguous = true;
+                continue;
+            }
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = t[a] ? 0 : 1;
+                hi[a] = t[a] ? 1 : nd[a];
+            }
+            for (int v2 = lo[2]; v2 <= hi[2]; ++v2)
+                for (int v1 = lo[1]; v1 <= hi[1]; ++v1)
+                    for (int v0 = lo[0]; v0 <= hi[0]; ++v0) {
+                        const int v[3] = {v0, v1, v2};
+                        int di[3], si[3];
+                        bool any = false;
+                        for (int a = 0; a < 3; ++a) {
+                            if (!t[a]) {
+                                di[a] = si[a] = v[a];
+                                continue;
+                            }
+                            const int q = v[a];
+                            any = any || q == 1;
+                            if (t[a] < 0) {
+                                di[a] = 1 - q;
+                                si[a] = ns[a] + 1 - q;
+                            } else {
+                                di[a] = nd[a] + q;
+                                si[a] = q;
+                            }
+                        }
+                        if (any) S.set(d, di[0], di[1], di[2], pre.get(s, si[0], si[1], si[2]));
+                    }
+        }
+        // faces: the interior rows of the ghost plane from the sender's
+        // boundary plane
+        for (int a = 0; a < 3; ++a)
+            for (int side = 0; side < 2; ++side) {
+                const int s = blk[d].nbr[2 * a + side];
+                if (s < 0) continue;
+                const int ns = ext_of(S.hb[s], a);
+                const int dst = side == 0 ? 0 : nd[a] + 1, src = side == 0 ? ns : 1;
+                const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+                for (int q = 1; q <= nd[o2]; ++q)
+                    for (int p = 1; p <= nd[o1]; ++p) {
+                        int di[3], si[3];
+                        plane_ijk(a, dst, p, q, di);
+                        plane_ijk(a, src, p, q, si);
+                        S.set(d, di[0], di[1], di[2], pre.get(s, si[0], si[1], si[2]));
+                    }
+            }
+    }
+    return !ambiguous;
+}
+
+void sym_pbound(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
+    for (int b = 0; b < (int)blk.size(); ++b) {
+        const BlockLevel &g = S.hb[b];
+        for (int a = 0; a < 3; ++a)
+            for (int side = 0; side < 2; ++side) {
+                if (!is_neumann(blk[b], 2 * a + side)) continue;
+                const int n = ext_of(g, a);
+                const int dst = side == 0 ? 0 : n + 1, src = side == 0 ? 1 : n;
+                const int e1 = ext_of(g, a == 0 ? 1 : 0) + 2, e2 = ext_of(g, a == 2 ? 1 : 2) + 2;
+                for (int q = 0; q < e2; ++q)
+                    for (int p = 0; p < e1; ++p) {
+                        int d[3], s[3];
+                        plane_ijk(a, dst, p, q, d);
+                        plane_ijk(a, src, p, q, s);
+                        S.set(b, d[0], d[1], d[2], S.get(b, s[0], s[1], s[2]));
+                    }
+            }
+    }
+}
+
+// Map entries {destination, source} of the ghosts the sequence changed.
+void map_entries(const GhostOrigins &S, std::vector<uint2> &out) {
+    for (int b = 0; b < (int)S.hb.size(); ++b) {
+        const BlockLevel &g = S.hb[b];
+        S.for_ghosts(b, [&](int i, int j, int k) {
+            const uint64_t self = GhostOrigins::self(g, i, j, k), org = S.get(b, i, j, k);
+            if (org != self) out.push_back(make_uint2((uint32_t)self, (uint32_t)org));
+        });
+    }
+}
+
+int build_vel(mf3d_mg *mg) {
+    if (mg->vel_ready) return 0;
+    const Level &L0 = mg->lv[0];
+    const int nb = mg->cfg.nblocks;
+    if (L0.cells_gh >= (1LL << 32)) {
+        mg->err = "velocity form: a level-0 field holds 2^32 entries or more";
+        return MF3D_ENOTSUP;
+    }
+    std::vector<VelBlock> vb(nb);
+    for (int b = 0; b < nb; ++b) {
+        const mf3d_block_desc &B = mg->blk[b];
+        for (int a = 0; a < 3; ++a) {
+            vb[b].ie[a] = face_end(B, a, a);
+            const int f = 2 * a;
+            if (B.nbr[f] >= 0) vb[b].lo_src[a] = B.nbr[f];
+            else if (B.bc[f] == 5) vb[b].lo_src[a] = B.nbr[f + 1] < 0 ? b : B.per[f];
+            else vb[b].lo_src[a] = -1;
+        }
+    }
+    const bool periodic = std::any_of(mg->blk.begin(), mg->blk.end(), [](const mf3d_block_desc &B) {
+        return std::any_of(B.bc, B.bc + 6, [](int c) { return c == 5; });
+    });
+    std::vector<uint2> ent;
+    bool ok = true;
+    for (int comp = 0; comp < 4; ++comp) {  // u, v, w, then p (cell field)
+        GhostOrigins S(L0.hb);
+        const int c = comp < 3 ? comp : -1;
+        if (periodic) sym_exchange_bc(mg->blk, S, c);
+        ok = sym_exchange_smlvl(mg->blk, S) && ok;
+        if (comp == 3) sym_pbound(mg->blk, S);
+        map_entries(S, ent);
+        if (comp < 3) mg->vmap_c[comp] = (long long)ent.size();
+    }
+    if (!ok) {
+        mg->err = "velocity form: edge or corner neighbours are ambiguous (block grid not conforming)";
+        return MF3D_ENOTSUP;
+    }
+    mg->vmap_p = (long long)ent.size() - mg->vmap_c[2];
+    int rc;
+    if ((rc = upload(mg, &mg->d_vb, vb)) || (rc = upload(mg, &mg->d_vmap, ent)) ||
+        (rc = dev_alloc(mg, (void **)&mg->d_vstage, sizeof(double) * std::max<size_t>(ent.size(), 1))))
+        return rc;
+    mg->vel_ready = true;
+    return 0;
+}
+
+void map_launch(mf3d_mg *mg, const uint2 *ent, MapArgs m, const int *done) {
+    const long long n = m.c[2];
+    if (n <= 0) return;
+    const unsigned gb = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_map_stage, dim3(gb), dim3(256), 0, mg->stream, m, ent, mg->d_vstage, done);
+    hipLaunchKernelGGL(k_map_scatter, dim3(gb), dim3(256), 0, mg->stream, m, ent, mg->d_vstage, done);
+}
+
+// One outer step of the velocity form after a V-cycle left pp in the
+// level-0 potential (update = true; newsolv_mg.for:38-55), or the step
+// before the loop (update = false; newsolv_mg.for:16-17): p += pp and
+// exchange(4) + pbound, then calvel + calmas and exchange(1..3), then rmax.
+template <typename T, bool VAR>
+void vel_step(mf3d_mg *mg, const VelFields &F, const int *done, bool update) {
+    Level &L0 = mg->lv[0];
+    if (update) {
+        const unsigned gb = (unsigned)std::min<long long>((L0.cells_gh + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_vel_pupdate<T>, dim3(gb), dim3(256), 0, mg->stream, (double *)F.p,
+                           (const T *)L0.phi[L0.cur], L0.cells_gh, done);
+        const long long np = mg->vmap_p;
+        map_launch(mg, mg->d_vmap + mg->vmap_c[2], MapArgs{{(double *)F.p, (double *)F.p, (double *)F.p}, {np, np, np}},
+                   done);
+    }
+    const bool tk = tk_on(mg, MF3D_K_OUTER, 0);
+    if (tk) tk_mark(mg);
+    hipLaunchKernelGGL((k_velmas<T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, F, (T *)L0.rhs,
+                       nullptr, mg->st, L0.db, mg->d_vb, mg->t_out, mg->partials, done);
+    if (tk) tk_mark(mg);
+    map_launch(mg, mg->d_vmap, MapArgs{{F.u, F.v, F.w}, {mg->vmap_c[0], mg->vmap_c[1], mg->vmap_c[2]}}, done);
+    hipLaunchKernelGGL(k_vel_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->n_out, mg->st,
+                       mg->d_rmax);
+}
+
+template <typename T, bool VAR>
+int newsolv_dev(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, double eps, double *su,
+                double *rmax_out, int32_t *iters_out, double *hist) {
+    int rc = build_vel(mg);
+    if (rc) return rc;
+    const int maxcy = mg->cfg.maxcy;
+    const bool use_graph = mg->tk == MF3D_K_OFF;
+    const void *key[7] = {F.us, F.vs, F.ws, F.u, F.v, F.w, F.p};
+    if (use_graph && (!mg->vgraph || std::memcmp(key, mg->vkey, sizeof(key)) != 0)) {
+        if (mg->vgraph) {
+            (void)hipGraphExecDestroy(mg->vgraph);
+            mg->vgraph = nullptr;
+        }
+        for (Level &V : mg->lv) V.cur = 0;
+        HIPCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeRelaxed));
+        vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
+        vel_step<T, VAR>(mg, F, done_ptr(mg), true);
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamEndCapture(mg->stream, &graph));
+        hipError_t e = hipGraphInstantiate(&mg->vgraph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            mg->vgraph = nullptr;
+            mg->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            return MF3D_EDEVICE;
+        }
+        std::memcpy(mg->vkey, key, sizeof(key));
+    }
+    mg->last_replay = use_graph;
+    hipLaunchKernelGGL(k_vel_reset, dim3(1), dim3(1), 0, mg->stream, mg->st, eps, dt * alfapr, dt);
+    vel_step<T, VAR>(mg, F, nullptr, false);  // calvel + calmas on the caller's p
+    int chunk = mg->guess > 0 ? mg->guess : 4, launched = 0;
+    SolveState hs{};
+    while (launched < maxcy) {
+        const int n = std::min(chunk, maxcy - launched);
+        for (int c = 0; c < n; ++c) {
+            if (use_graph) {
+                HIPCHK(hipGraphLaunch(mg->vgraph, mg->stream));
+            } else {
+                for (Level &V : mg->lv) V.cur = 0;
+                vcycle_dev<T, VAR>(mg, true, done_ptr(mg));
+                vel_step<T, VAR>(mg, F, done_ptr(mg), true);
+            }
+        }
+        launched += n;
+        HIPCHK(hipMemcpyAsync(&hs, mg->st, sizeof(SolveState), hipMemcpyDeviceToHost, mg->stream));
+        HIPCHK(hipStreamSynchronize(mg->stream));
+        tk_flush(mg);
+        if (hs.done) break;
+        chunk = 1;
+    }
+    const int k = hs.last;
+    if (k < 0) {
+        mg->err = "no outer step recorded";
+        return MF3D_EDEVICE;
+    }
+    mg->guess = std::max(k, 1);
+    if (su) {  // the last calmas once more, its su in FP64 (faces rewritten unchanged)
+        Level &L0 = mg->lv[0];
+        hipLaunchKernelGGL((k_velmas<double, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, F,
+                           (double *)nullptr, su, mg->st, L0.db, mg->d_vb, mg->t_out, (double *)nullptr,
+                           (const int *)nullptr);
+    }
+    std::vector<double> rm(k + 1);
+    HIPCHK(hipMemcpyAsync(rm.data(), mg->d_rmax, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, mg->stream));
+    HIPCHK(hipStreamSynchronize(mg->stream));
+    HIPCHK(hipGetLastError());
+    if (hist) std::memcpy(hist, rm.data(), sizeof(double) * (k + 1));
+    if (iters_out) *iters_out = k;
+    if (rmax_out) *rmax_out = rm[k];
+    if (!(rm[k] <= 100.0)) {  // newsolv_mg.for:64 (a NaN counts as diverged here)
+        mg->err = "BIG RMAX (rmax > 100): diverged";
+        return MF3D_DIVERGED;
+    }
+    return hs.done ? MF3D_OK : MF3D_NOT_CONVERGED;
+}
+
+int newsolv_any(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, double eps, double *su,
+                double *rmax_out, int32_t *iters_out, double *hist) {
+    if (mg->cfg.variable_coef && !mg->dens_ghosted) {
+        mg->err = "velocity form under LSM needs the density with its ghost layer (mf3d_mg_set_density "
+                  "with ghost >= 1)";
+        return MF3D_EINVAL;
+    }
+    const bool mixed = mg->cfg.precision == MF3D_MIXED, var = mg->cfg.variable_coef != 0;
+    if (mixed)
+        return var ? newsolv_dev<float, true>(mg, F, dt, alfapr, eps, su, rmax_out, iters_out, hist)
+                   : newsolv_dev<float, false>(mg, F, dt, alfapr, eps, su, rmax_out, iters_out, hist);
+    return var ? newsolv_dev<double, true>(mg, F, dt, alfapr, eps, su, rmax_out, iters_out, hist)
+               : newsolv_dev<double, false>(mg, F, dt, alfapr, eps, su, rmax_out, iters_out, hist);
 }
 
 }  // namespace
@@ -1168,11 +1627,21 @@ int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks) {
     }
     int rc = set_device(mg);
     if (rc) return rc;
-    host_to_packed(mg, dens_blocks, mg->h_io, 0);
-    HIPCHK(hipMemcpyAsync(mg->d_io, mg->h_io, sizeof(double) * mg->cells, hipMemcpyHostToDevice, mg->stream));
     Level &L0 = mg->lv[0];
-    hipLaunchKernelGGL(k_unpack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->d_io, L0.dens,
-                       L0.db, L0.t_in);
+    // with a ghost layer in the caller's arrays the level-0 density keeps it
+    // (the velocity form's calvel reads rho across block faces); the
+    // coefficient hierarchy reads interior densities only
+    mg->dens_ghosted = std::all_of(mg->blk.begin(), mg->blk.end(),
+                                   [](const mf3d_block_desc &B) { return B.ghost >= 1; });
+    if (mg->dens_ghosted) {
+        host_to_packed(mg, dens_blocks, mg->h_io, 1);
+        HIPCHK(hipMemcpyAsync(L0.dens, mg->h_io, sizeof(double) * L0.cells_gh, hipMemcpyHostToDevice, mg->stream));
+    } else {
+        host_to_packed(mg, dens_blocks, mg->h_io, 0);
+        HIPCHK(hipMemcpyAsync(mg->d_io, mg->h_io, sizeof(double) * mg->cells, hipMemcpyHostToDevice, mg->stream));
+        hipLaunchKernelGGL(k_unpack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->d_io, L0.dens,
+                           L0.db, L0.t_in);
+    }
     return build_lsm(mg);
 }
 
@@ -1244,6 +1713,67 @@ int mf3d_mg_solve_device(mf3d_mg *mg, const double *f_dev, double *p_dev, void *
     mg->stream = own;
     return rc;
 }
+
+int mf3d_mg_newsolv_device(mf3d_mg *mg, const double *ustar, const double *vstar, const double *wstar, double *u,
+                           double *v, double *w, double *p, double dt, double alfapr, double eps, double *su,
+                           void *stream, double *rmax_out, int32_t *iters_out, double *rmax_hist) {
+    if (!mg || !ustar || !vstar || !wstar || !u || !v || !w || !p) return MF3D_EINVAL;
+    int rc = check_ready(mg);
+    if (rc) return rc;
+    if ((rc = set_device(mg))) return rc;
+    const VelFields F{ustar, vstar, wstar, u, v, w, p, mg->cfg.variable_coef ? mg->lv[0].dens : nullptr};
+    hipStream_t own = mg->stream;
+    if (stream) mg->stream = (hipStream_t)stream;
+    rc = newsolv_any(mg, F, dt, alfapr, eps, su, rmax_out, iters_out, rmax_hist);
+    mg->stream = own;
+    return rc;
+}
+
+int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const *vstar, const double *const *wstar,
+                    double *const *u, double *const *v, double *const *w, double *const *p, double dt,
+                    double alfapr, double eps, double *const *su, double *rmax_out, int32_t *iters_out,
+                    double *rmax_hist) {
+    if (!mg || !ustar || !vstar || !wstar || !u || !v || !w || !p) return MF3D_EINVAL;
+    int rc = check_ready(mg);
+    if (rc) return rc;
+    for (int b = 0; b < mg->cfg.nblocks; ++b)
+        if (mg->blk[b].ghost < 1) {
+            mg->err = "newsolv needs caller arrays with ghost >= 1";
+            return MF3D_EINVAL;
+        }
+    if ((rc = set_device(mg))) return rc;
+    Level &L0 = mg->lv[0];
+    const size_t bytes = sizeof(double) * (size_t)L0.cells_gh;
+    for (int i = 0; i < 8; ++i)
+        if (!mg->vf[i] && (i < 7 || su) && (rc = dev_alloc(mg, (void **)&mg->vf[i], bytes))) return rc;
+    const double *const *in[7] = {ustar, vstar, wstar, u, v, w, p};
+    for (int i = 0; i < 7; ++i) {
+        host_to_packed(mg, in[i], mg->h_io, 1);
+        HIPCHK(hipMemcpyAsync(mg->vf[i], mg->h_io, bytes, hipMemcpyHostToDevice, mg->stream));
+        HIPCHK(hipStreamSynchronize(mg->stream));
+    }
+    const VelFields F{mg->vf[0], mg->vf[1], mg->vf[2], mg->vf[3], mg->vf[4], mg->vf[5], mg->vf[6],
+                      mg->cfg.variable_coef ? L0.dens : nullptr};
+    rc = newsolv_any(mg, F, dt, alfapr, eps, su ? mg->vf[7] : nullptr, rmax_out, iters_out, rmax_hist);
+    if (rc < 0) return rc;
+    double *const *out[4] = {u, v, w, p};
+    for (int i = 0; i < 4; ++i) {
+        HIPCHK(hipMemcpyAsync(mg->h_io, mg->vf[3 + i], bytes, hipMemcpyDeviceToHost, mg->stream));
+        HIPCHK(hipStreamSynchronize(mg->stream));
+        packed_to_host(mg, mg->h_io, out[i], 1);
+    }
+    if (su) {
+        hipLaunchKernelGGL(k_pack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->vf[7], mg->d_io,
+                           L0.db, L0.t_in);
+        HIPCHK(hipMemcpyAsync(mg->h_io, mg->d_io, sizeof(double) * mg->cells, hipMemcpyDeviceToHost, mg->stream));
+        HIPCHK(hipStreamSynchronize(mg->stream));
+        packed_to_host(mg, mg->h_io, su, 0);
+    }
+    HIPCHK(hipGetLastError());
+    return rc;
+}
+
+int64_t mf3d_mg_ghosted_cells(const mf3d_mg *mg) { return mg ? mg->lv[0].cells_gh : MF3D_EINVAL; }
 
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel) {
     if (!mg || kernel < MF3D_K_OFF || kernel > MF3D_K_PROLONG) return MF3D_EINVAL;

@@ -61,6 +61,7 @@ struct SolveState {
     int fk;     // GCR steps taken
     double tol; // stop tolerance of this solve (read by captured cycles)
     double alpha, beta, gamma;  // GCR: step, direction weight, (w, w)
+    double dta, dt;             // velocity form: dt*alfapr and dt of this solve
 };
 
 __device__ __forceinline__ long long gidx(const BlockLevel &g, int i, int j, int k) {

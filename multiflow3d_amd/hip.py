@@ -63,6 +63,12 @@ class DeviceArray:
         _check(rt().hipMemcpy(d.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
         return d
 
+    def copy_from_host(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != self.n:
+            raise ValueError(f"size {a.size} != {self.n}")
+        _check(rt().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
     def to_host(self) -> np.ndarray:
         out = np.empty(self.n)
         _check(rt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, D2H), "hipMemcpy D2H")

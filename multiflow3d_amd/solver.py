@@ -5,7 +5,10 @@ The reference exposes three argument-less subroutines over module state:
 
     coeff        mgsolver.for:2-222      coefficient hierarchy
     mgkcyc       mgsolver.for:224-369    one V-cycle on su (sup) into pp
-    newsolv_mg   newsolv_mg.for:2-102    outer defect-correction loop
+    newsolv_mg   newsolv_mg.for:2-102    outer loop: the reference's velocity
+                                         form (calvel/calmas/pbound, press.for)
+                                         as ``newsolv``, or the Poisson form
+                                         f -> p as ``newsolv_mg``
 
 ``PressureMultigrid`` keeps those names; the module state becomes the
 constructor's arguments (a block grid and the MG scalars of control.cin)
@@ -221,6 +224,65 @@ class PressureMultigrid:
                 else:
                     dst[1:-1, 1:-1, 1:-1] = src
         return out, res
+
+    @property
+    def ghosted_cells(self) -> int:
+        """Length of each device field of newsolv_device (all blocks' ghosted arrays)."""
+        return int(_lib.lib().mf3d_mg_ghosted_cells(self._h))
+
+    def _vel_arrays(self, arrays, name, writable):
+        out = []
+        for b, a in zip(self.blocks, arrays):
+            shp = (b.n[2] + 2, b.n[1] + 2, b.n[0] + 2)
+            if a.shape != shp or a.dtype != np.float64 or not a.flags.c_contiguous or \
+                    (writable and not a.flags.writeable):
+                raise ValueError(f"{name} blocks must be C-contiguous float64 arrays shaped {shp}")
+            out.append(a)
+        if len(out) != len(self.blocks):
+            raise ValueError(f"{name}: one array per block")
+        return out
+
+    def newsolv(self, ustar, vstar, wstar, u, v, w, p, dt: float, alfapr: float, eps: float,
+                su: Optional[list] = None, warn: bool = True) -> SolveResult:
+        """newsolv_mg in the reference's velocity form (newsolv_mg.for:12-58):
+        calvel/calmas, then V-cycles on su (sup under LSM), p += pp,
+        exchange(4)/pbound, calvel/calmas, until rmax < eps after two cycles.
+        All fields are per-block ghosted arrays (nz+2, ny+2, nx+2); u, v, w and
+        p are updated in place (faces on walls and untouched ghosts keep their
+        values), su (optional, same shape) gets the last right-hand side in its
+        interior.  Returns a SolveResult whose hist is rmax per cycle (index 0
+        before the first).  Under LSM, coeff(density) must have received the
+        ghosted density."""
+        arrs = [self._vel_arrays(x, nm, wr) for x, nm, wr in
+                ((ustar, "ustar", False), (vstar, "vstar", False), (wstar, "wstar", False),
+                 (u, "u", True), (v, "v", True), (w, "w", True), (p, "p", True))]
+        sup = None
+        if su is not None:
+            sup = self._ptrs(self._vel_arrays(su, "su", True))
+        rmax = ctypes.c_double(0.0)
+        iters = ctypes.c_int32(0)
+        self._hist = np.zeros(self.maxcy + 1)
+        rc = _lib.lib().mf3d_mg_newsolv(
+            self._h, *[self._ptrs(a) for a in arrs], float(dt), float(alfapr), float(eps), sup,
+            ctypes.byref(rmax), ctypes.byref(iters),
+            self._hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return self._finish(rc, rmax, iters, warn)
+
+    def newsolv_device(self, ptrs, dt: float, alfapr: float, eps: float, su_ptr: int = 0,
+                       stream_ptr: int = 0, warn: bool = True) -> SolveResult:
+        """Device-resident velocity form: ptrs = (ustar, vstar, wstar, u, v, w, p)
+        device addresses of ghosted_cells doubles each (e.g. torch tensors'
+        data_ptr()); su_ptr optional."""
+        if len(ptrs) != 7:
+            raise ValueError("ptrs: ustar, vstar, wstar, u, v, w, p")
+        rmax = ctypes.c_double(0.0)
+        iters = ctypes.c_int32(0)
+        self._hist = np.zeros(self.maxcy + 1)
+        rc = _lib.lib().mf3d_mg_newsolv_device(
+            self._h, *[ctypes.c_void_p(int(x)) for x in ptrs], float(dt), float(alfapr), float(eps),
+            ctypes.c_void_p(su_ptr or None), ctypes.c_void_p(stream_ptr or None), ctypes.byref(rmax),
+            ctypes.byref(iters), self._hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return self._finish(rc, rmax, iters, warn)
 
     def time_kernel(self, kernel: int):
         """Enable the finest-level timing hook for one kernel (MF3D_K_*; 0 = off)."""
