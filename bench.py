@@ -95,6 +95,59 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     return out
 
 
+def run_velocity(grid, levels, precision, steps, warmup, rel_eps, maxcy, time_kernel):
+    """The velocity form of newsolv_mg (calvel/calmas around each V-cycle,
+    the reference's own outer loop) on resident u*, v*, w*: each step
+    starts from u = u*, p = 0 (restored outside the timed call) and runs
+    until rmax < rel_eps * rmax_0.  Steps are timed one by one on the host
+    around the synchronous call."""
+    import torch
+
+    from multiflow3d_amd import PressureMultigrid
+    from multiflow3d_amd import _lib
+    from multiflow3d_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    us, vs, ws = W.velocity_inputs(grid, W.BASE_SEED + 40)
+    init = [torch.from_numpy(np.ascontiguousarray(np.concatenate([a.ravel() for a in x]))).to(dev)
+            for x in (us, vs, ws)]
+    work = [t.clone() for t in init] + [torch.zeros_like(init[0])]
+    torch.cuda.synchronize()
+    mg = PressureMultigrid(grid, levels, maxcy=maxcy, precision=precision)
+
+    def reset():
+        for dst, src in zip(work[:3], init):
+            dst.copy_(src)
+        work[3].zero_()
+        torch.cuda.synchronize()
+
+    ptrs = [t.data_ptr() for t in init] + [t.data_ptr() for t in work]
+    reset()
+    r0 = mg.newsolv_device(ptrs, 0.01, 1.0, 0.0, warn=False)  # rmax_0 from step 0's calmas
+    eps = rel_eps * float(r0.hist[0])
+    if time_kernel:
+        mg.time_kernel(_lib.MF3D_K_OUTER)
+    for _ in range(warmup):
+        reset()
+        mg.newsolv_device(ptrs, 0.01, 1.0, eps, warn=False)
+    if time_kernel:
+        mg.time_kernel(_lib.MF3D_K_OUTER)
+    lat = []
+    res = None
+    for _ in range(steps):
+        reset()
+        ts = time.perf_counter()
+        res = mg.newsolv_device(ptrs, 0.01, 1.0, eps, warn=False)
+        lat.append((time.perf_counter() - ts) * 1e3)
+    launches, kms = mg.kernel_time() if time_kernel else (0, 0.0)
+    replay = mg.graph_replay()
+    mg.time_kernel(0)
+    u = work[0].cpu().numpy()
+    mg.close()
+    return dict(ms=float(np.mean(lat)), lat=lat, iters=res.iters, rmax=res.rmax, rmax0=float(r0.hist[0]),
+                eps=eps, status=res.status, u=u, launches=launches, kernel_ms=kms, graph_replay=replay)
+
+
 def run_serving(count):
     """Serving sweep: Poisson arrivals of independent solves, n^3 with n
     uniform in {32,64,96,128,192,256}, tolerance uniform in {1e-6,1e-8,1e-10},
@@ -137,6 +190,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxcy", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-velocity", action="store_true",
+                    help="skip the velocity-form (calvel/calmas) leg")
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests per arrival rate in the serving sweep (0: skip it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -236,6 +291,38 @@ def main():
                          f"single thread",
                "fp64_gpu_bitwise_equal": fp64_bitwise}
 
+    velocity = None
+    if not args.no_velocity:
+        vel = {}
+        for prec in ("fp64", "mixed"):
+            v = run_velocity(grid, L, prec, args.steps, args.warmup, args.tol, args.maxcy, False)
+            t = run_velocity(grid, L, prec, 2, 1, args.tol, args.maxcy, True)  # eager, kernel hook on
+            w = 8 if prec == "fp64" else 4
+            per_launch = (56 + w) * N0
+            avg_s = t["kernel_ms"] / t["launches"] * 1e-3 if t["launches"] else None
+            vel[prec] = {"solves_per_s": round(1e3 / v["ms"], 4), "ms_per_solve": round(v["ms"], 4),
+                         "cycles": v["iters"], "rmax0": v["rmax0"], "eps": v["eps"], "final_rmax": v["rmax"],
+                         "status": v["status"], "graph_replay": v["graph_replay"],
+                         "p50_ms": round(float(np.percentile(v["lat"], 50)), 3),
+                         "p99_ms": round(float(np.percentile(v["lat"], 99)), 3),
+                         "k_velmas": None if avg_s is None else {
+                             "bound": "hbm", "avg_us": round(avg_s * 1e6, 2), "launches_timed": t["launches"],
+                             "bytes_per_launch": int(per_launch),
+                             "achieved": round(per_launch / avg_s / 1e9, 1), "peak": HBM_PEAK_GBPS,
+                             "unit": "GB/s", "frac": round(per_launch / avg_s / 1e9 / HBM_PEAK_GBPS, 4),
+                             "model": "reads u*, v*, w*, p (32 B), writes u, v, w (24 B) and the "
+                                      "V-cycle RHS (w B) per cell"}}
+            vel[prec + "_u"] = v["u"]
+            print(f"# velocity {prec}: {v['ms']:.3f} ms/solve, {v['iters']} cycles, rmax {v['rmax']:.3e} "
+                  f"(eps {v['eps']:.3e})", file=sys.stderr, flush=True)
+        ua, ub = vel.pop("mixed_u"), vel.pop("fp64_u")
+        vel["mixed_u_rel_l2_vs_fp64"] = float(np.linalg.norm(ua - ub) / np.linalg.norm(ub))
+        vel["speedup_mixed_vs_fp64"] = round(vel["fp64"]["ms_per_solve"] / vel["mixed"]["ms_per_solve"], 3)
+        vel["workload"] = (f"newsolv_mg velocity form (calvel/calmas/exchange around each V-cycle) on the "
+                           f"same {n}^3 channel, u* seeded, p from 0, dt 0.01, alfapr 1, until "
+                           f"rmax < {args.tol:g} x rmax_0")
+        velocity = vel
+
     serving = run_serving(args.serving_requests) if args.serving_requests > 0 else None
 
     lat = np.array(rmx["lat"])
@@ -271,6 +358,7 @@ def main():
                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
                   "rel_l2_vs_fp64": mixed_err, "graph_replay": rmx["graph_replay"]},
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+        "velocity_form": velocity,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -47,11 +47,13 @@ def build(verbose: bool = False) -> str:
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_HOST = os.path.join(FORTRAN_DIR, "_build", "mf3d_host")
+FORTRAN_HOST_VEL = os.path.join(FORTRAN_DIR, "_build", "mf3d_host_vel")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """The ISO_C_BINDING module and the small Fortran host that drives the
-    library through it (fortran/_build/mf3d_host, linked to libmf3d_mg.so)."""
+    """The ISO_C_BINDING module and the small Fortran hosts that drive the
+    library through it (fortran/_build/mf3d_host for the Poisson form,
+    mf3d_host_vel for the velocity form; linked to libmf3d_mg.so)."""
     out = os.path.join(FORTRAN_DIR, "_build")
     os.makedirs(out, exist_ok=True)
     steps = [
@@ -60,6 +62,10 @@ def build_fortran(verbose: bool = False) -> str:
         [FLANG, "-c", "-O2", "-J", out, "-I", out, os.path.join(FORTRAN_DIR, "mf3d_host.f90"),
          "-o", os.path.join(out, "mf3d_host.o")],
         [FLANG, "-o", FORTRAN_HOST, os.path.join(out, "mf3d_host.o"), os.path.join(out, "mf3d_mg_f.o"),
+         f"-L{HERE}", "-lmf3d_mg", "-Wl,-rpath,$ORIGIN/../.."],
+        [FLANG, "-c", "-O2", "-J", out, "-I", out, os.path.join(FORTRAN_DIR, "mf3d_host_vel.f90"),
+         "-o", os.path.join(out, "mf3d_host_vel.o")],
+        [FLANG, "-o", FORTRAN_HOST_VEL, os.path.join(out, "mf3d_host_vel.o"), os.path.join(out, "mf3d_mg_f.o"),
          f"-L{HERE}", "-lmf3d_mg", "-Wl,-rpath,$ORIGIN/../.."],
     ]
     for cmd in steps:

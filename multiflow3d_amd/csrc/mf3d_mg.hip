@@ -129,6 +129,11 @@ struct mf3d_mg {
     long long vmap_p = 0;
     double *d_vstage = nullptr;
     double *vf[8] = {};  // u*, v*, w*, u, v, w, p, su
+    int vel_wg = 0;      // persistent grid of k_velmas (tiles spread evenly over resident workgroups)
+    Tile *t_vel = nullptr;  // its tiles: 64 x 8 columns, chunks of at most VEL_TZ_MAX planes
+    double *d_vpart = nullptr;  // its per-workgroup maxima
+    double *d_vsink = nullptr;  // one slot per thread of its grid for stores a lane must not make
+    int n_vel = 0;
     hipGraphExec_t vgraph = nullptr;
     const void *vkey[7] = {};
 };
@@ -1448,6 +1453,11 @@ int build_vel(mf3d_mg *mg) {
     }
     mg->vmap_p = (long long)ent.size() - mg->vmap_c[2];
     int rc;
+    {
+        auto tv = xcd_order(sweep_tiles(L0.hb, 8, VEL_TZ_MAX));
+        mg->n_vel = (int)tv.size();
+        if ((rc = upload(mg, &mg->t_vel, tv))) return rc;
+    }
     if ((rc = upload(mg, &mg->d_vb, vb)) || (rc = upload(mg, &mg->d_vmap, ent)) ||
         (rc = dev_alloc(mg, (void **)&mg->d_vstage, sizeof(double) * std::max<size_t>(ent.size(), 1))))
         return rc;
@@ -1480,19 +1490,36 @@ void vel_step(mf3d_mg *mg, const VelFields &F, const int *done, bool update) {
     }
     const bool tk = tk_on(mg, MF3D_K_OUTER, 0);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL((k_velmas<T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, F, (T *)L0.rhs,
-                       nullptr, mg->st, L0.db, mg->d_vb, mg->t_out, mg->partials, done);
+    hipLaunchKernelGGL((k_velmas<T, VAR, false, 8>), dim3(mg->vel_wg), dim3(SW_TX, 8), 0, mg->stream, F,
+                       (T *)L0.rhs, (double *)nullptr, mg->d_vsink, mg->st, L0.db, mg->d_vb, mg->t_vel, mg->n_vel,
+                       mg->d_vpart, done);
     if (tk) tk_mark(mg);
     map_launch(mg, mg->d_vmap, MapArgs{{F.u, F.v, F.w}, {mg->vmap_c[0], mg->vmap_c[1], mg->vmap_c[2]}}, done);
-    hipLaunchKernelGGL(k_vel_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->n_out, mg->st,
+    hipLaunchKernelGGL(k_vel_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->d_vpart, mg->vel_wg, mg->st,
                        mg->d_rmax);
+}
+
+// Workgroups of k_velmas resident at once on the device (its persistent
+// grid), at most one per tile.
+template <typename T, bool VAR>
+int vel_grid(mf3d_mg *mg) {
+    if (mg->vel_wg) return 0;
+    int per_cu = 0, cus = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_velmas<T, VAR, false, 8>, SW_TX * 8, 0));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, mg->cfg.device));
+    const int cap = std::max(1, per_cu) * std::max(1, cus);
+    const int rounds = (mg->n_vel + cap - 1) / cap;
+    mg->vel_wg = std::max(1, (mg->n_vel + rounds - 1) / rounds);  // every workgroup walks `rounds` tiles or one fewer
+    int rc = dev_alloc(mg, (void **)&mg->d_vpart, sizeof(double) * (size_t)mg->vel_wg);
+    if (!rc) rc = dev_alloc(mg, (void **)&mg->d_vsink, sizeof(double) * (size_t)mg->vel_wg * SW_TX * 8);
+    return rc;
 }
 
 template <typename T, bool VAR>
 int newsolv_dev(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, double eps, double *su,
                 double *rmax_out, int32_t *iters_out, double *hist) {
     int rc = build_vel(mg);
-    if (rc) return rc;
+    if (rc || (rc = vel_grid<T, VAR>(mg))) return rc;
     const int maxcy = mg->cfg.maxcy;
     const bool use_graph = mg->tk == MF3D_K_OFF;
     const void *key[7] = {F.us, F.vs, F.ws, F.u, F.v, F.w, F.p};
@@ -1547,9 +1574,9 @@ int newsolv_dev(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, doubl
     mg->guess = std::max(k, 1);
     if (su) {  // the last calmas once more, its su in FP64 (faces rewritten unchanged)
         Level &L0 = mg->lv[0];
-        hipLaunchKernelGGL((k_velmas<double, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream, F,
-                           (double *)nullptr, su, mg->st, L0.db, mg->d_vb, mg->t_out, (double *)nullptr,
-                           (const int *)nullptr);
+        hipLaunchKernelGGL((k_velmas<double, VAR, true, 8>), dim3(mg->vel_wg), dim3(SW_TX, 8), 0, mg->stream, F,
+                           (double *)nullptr, su, mg->d_vsink, mg->st, L0.db, mg->d_vb, mg->t_vel, mg->n_vel,
+                           (double *)nullptr, (const int *)nullptr);
     }
     std::vector<double> rm(k + 1);
     HIPCHK(hipMemcpyAsync(rm.data(), mg->d_rmax, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, mg->stream));

@@ -79,106 +79,212 @@ struct VelFields {
     const double *dens;  // cell density (LSM), ghosted
 };
 
-// One face of component A of block g at the cell with arena offset c and
-// index i along A: computed from u* and p when i <= ie, else the stored
-// face (a wall face the caller owns).
-template <int A, bool LSM>
-__device__ __forceinline__ double vel_face(const VelFields &F, const BlockLevel &g, int ie, long long c, int i,
+// Operands of one face: value = s - (dt*alfapr*(p1 - p0))/h, divided by
+// 0.5*(d0 + d1) under LSM (press.for:150-188).  A face the block does not
+// compute (a wall face the caller owns) is loaded as s with `stored` set.
+struct FaceOp {
+    double s, p0, p1, d0, d1, h;
+    bool stored;
+};
+
+template <bool LSM>
+__device__ __forceinline__ double face_val(double s, double p0, double p1, double d0, double d1, double h,
                                            double dta) {
-    const double *st = A == 0 ? F.us : (A == 1 ? F.vs : F.ws);
-    const double *cur = A == 0 ? F.u : (A == 1 ? F.v : F.w);
-    if (i > ie) return cur[c];
-    const long long s = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
-    const double h = A == 0 ? g.h0 : (A == 1 ? g.h1 : g.h2);
-    double gr = (dta * (F.p[c + s] - F.p[c])) / h;
-    if (LSM) gr = gr / (0.5 * (F.dens[c] + F.dens[c + s]));
-    return st[c] - gr;
+    double gr = (dta * (p1 - p0)) / h;
+    if (LSM) gr = gr / (0.5 * (d0 + d1));
+    return s - gr;
+}
+
+template <bool LSM>
+__device__ __forceinline__ double face_val(const FaceOp &f, double dta) {
+    return f.stored ? f.s : face_val<LSM>(f.s, f.p0, f.p1, f.d0, f.d1, f.h, dta);
+}
+
+// Operands of face A of block g at arena offset c, index i along A.
+template <int A, bool LSM>
+__device__ __forceinline__ FaceOp own_face(const VelFields &F, const BlockLevel &g, int ie, long long c, int i) {
+    FaceOp f;
+    const long long st = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
+    f.h = A == 0 ? g.h0 : (A == 1 ? g.h1 : g.h2);
+    f.stored = i > ie;
+    const double *sv = f.stored ? (A == 0 ? F.u : (A == 1 ? F.v : F.w)) : (A == 0 ? F.us : (A == 1 ? F.vs : F.ws));
+    f.s = sv[c];
+    f.p0 = F.p[c];
+    f.p1 = F.p[c + st];
+    f.d0 = LSM ? F.dens[c] : 1.0;
+    f.d1 = LSM ? F.dens[c + st] : 1.0;
+    return f;
 }
 
 // The ghost face below the first cell of component A at transverse cell
-// (i1, i2) (the two other indices, in x, y, z order): the source block's
-// last face, recomputed exactly as that block computes it, or the stored
+// (i1, i2) (the two other indices in x, y, z order): the source block's
+// last face, computed exactly as that block computes it, or the stored
 // face before a wall.
 template <int A, bool LSM>
-__device__ __forceinline__ double vel_low_face(const VelFields &F, const BlockLevel *__restrict__ bls,
-                                               const VelBlock *__restrict__ vbs, const BlockLevel &g, int b,
-                                               int i1, int i2, double dta) {
+__device__ __forceinline__ FaceOp low_face(const VelFields &F, const BlockLevel *__restrict__ bls,
+                                           const VelBlock *__restrict__ vbs, const BlockLevel &g, int b, int i1,
+                                           int i2) {
     const int s = vbs[b].lo_src[A];
     if (s < 0) {
-        const double *cur = A == 0 ? F.u : (A == 1 ? F.v : F.w);
+        FaceOp f;
         const long long c0 = A == 0 ? gidx(g, 0, i1, i2) : (A == 1 ? gidx(g, i1, 0, i2) : gidx(g, i1, i2, 0));
-        return cur[c0];
+        f.s = (A == 0 ? F.u : (A == 1 ? F.v : F.w))[c0];
+        f.p0 = f.p1 = 0.0;
+        f.d0 = f.d1 = 1.0;
+        f.h = 1.0;
+        f.stored = true;
+        return f;
     }
     const BlockLevel gs = bls[s];
     const int ns = A == 0 ? gs.n0 : (A == 1 ? gs.n1 : gs.n2);
     const long long cs = A == 0 ? gidx(gs, ns, i1, i2) : (A == 1 ? gidx(gs, i1, ns, i2) : gidx(gs, i1, i2, ns));
-    return vel_face<A, LSM>(F, gs, vbs[s].ie[A], cs, ns, dta);
+    return own_face<A, LSM>(F, gs, vbs[s].ie[A], cs, ns);
 }
 
-// calvel fused with calmas over the finest level: each thread owns one
-// (x, y) column of a 64 x TY tile and marches its z chunk.  Writes the
-// faces the block computes (its own east, north and top faces), the
-// V-cycle right-hand side su (sup under LSM) in the inner precision TR,
-// optionally su in FP64 (su64), and per workgroup max|fact*dx*dy*dz|.
-// The west face of a column comes from the lane to its left, the south
-// face is recomputed, the bottom face is carried from the plane below.
-template <typename TR, bool LSM, int TY>
-__global__ __launch_bounds__(SW_TX *TY) void k_velmas(VelFields F, TR *__restrict__ rhs, double *__restrict__ su64,
-                                                      const SolveState *__restrict__ st,
-                                                      const BlockLevel *__restrict__ bls,
-                                                      const VelBlock *__restrict__ vbs,
-                                                      const Tile *__restrict__ tiles, double *__restrict__ partials,
-                                                      const int *__restrict__ done) {
+// calvel fused with calmas over the finest level.  A persistent grid walks
+// the tile table (64 x TY columns, a chunk of planes); each thread owns one
+// (x, y) column and marches the chunk.  Per tile the faces west of the
+// tile's first column and south of its first row are computed up front
+// into LDS (the block's own faces, a source block's last face, or the
+// caller's wall face).  Per plane each thread computes its own east, north
+// and top faces; the west face comes from the lane to its left, the south
+// face from the row below through LDS, the bottom face from the plane
+// below.  Writes the faces the block computes, the right-hand side su (sup
+// under LSM) -- into the V-cycle's RHS in the inner precision TR, or in
+// FP64 into su64 when SU64 -- and per workgroup max|fact*dx*dy*dz|.
+//
+// Every plane issues the same memory operations: the next plane's loads
+// (the last plane of a chunk loads its own plane again) go out before this
+// plane's stores, in two register sets used in turn, and a store a lane
+// must not make (a wall face, a column outside the block) goes to the
+// lane's own slot of `sink` instead.  With no path-dependent loads or
+// stores the wait for a set covers exactly that set.
+constexpr int VEL_TZ_MAX = 32;
+
+template <typename TR, bool LSM, bool SU64, int TY>
+__global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
+    VelFields F, TR *__restrict__ rhs, double *__restrict__ su64, double *__restrict__ sink,
+    const SolveState *__restrict__ st, const BlockLevel *__restrict__ bls, const VelBlock *__restrict__ vbs,
+    const Tile *__restrict__ tiles, int ntiles, double *__restrict__ partials, const int *__restrict__ done) {
     if (done && *done) return;
+    constexpr int NT = SW_TX * TY;
+    __shared__ double sW[VEL_TZ_MAX][TY];      // west faces of the tile's first column
+    __shared__ double sS[VEL_TZ_MAX][SW_TX];   // south faces of the tile's first row
+    __shared__ double sN[2][TY][SW_TX];        // north faces of each row, per plane parity
     const double dta = st->dta, dt = st->dt;
-    const Tile t = tiles[blockIdx.x];
-    const BlockLevel g = bls[t.b];
-    const VelBlock vb = vbs[t.b];
-    const int x = t.x0 + threadIdx.x, y = t.y0 + threadIdx.y;
-    const bool act = x <= g.n0 && y <= g.n1;
-    // inactive lanes work on a clamped column so the lane shuffle sees
-    // every lane; nothing they compute is stored
-    const int xc = act ? x : 1, yc = act ? y : 1;
-    const double dx = g.h0, dy = g.h1, dz = g.h2;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + SW_TX * ty;
+    double *const my_sink = sink + (long long)blockIdx.x * NT + tid;
     double rm = 0.0;
-    double wB;
-    {
-        const int z = t.z0;
-        if (z == 1) wB = vel_low_face<2, LSM>(F, bls, vbs, g, t.b, xc, yc, dta);
-        else wB = vel_face<2, LSM>(F, g, vb.ie[2], gidx(g, xc, yc, z - 1), z - 1, dta);
-    }
-    const int lane = threadIdx.x;  // SW_TX == wavefront size: one row per wave
-    for (int z = t.z0; z <= t.z1; ++z) {
-        const long long c = gidx(g, xc, yc, z);
-        const double uE = vel_face<0, LSM>(F, g, vb.ie[0], c, xc, dta);
-        const double vN = vel_face<1, LSM>(F, g, vb.ie[1], c, yc, dta);
-        const double wT = vel_face<2, LSM>(F, g, vb.ie[2], c, z, dta);
-        double uW = __shfl_up(uE, 1);
-        if (lane == 0) {
-            uW = xc == 1 ? vel_low_face<0, LSM>(F, bls, vbs, g, t.b, yc, z, dta)
-                         : vel_face<0, LSM>(F, g, vb.ie[0], c - 1, xc - 1, dta);
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile t = tiles[ti];
+        const BlockLevel g = bls[t.b];
+        const VelBlock vb = vbs[t.b];
+        const int nz = t.z1 - t.z0 + 1;
+        __syncthreads();  // the previous tile's halo reads are done
+        // west halo: TY rows x nz planes
+        for (int q = tid; q < TY * nz; q += NT) {
+            const int yy = t.y0 + q % TY, zz = t.z0 + q / TY;
+            if (yy <= g.n1) {
+                const FaceOp f = t.x0 == 1 ? low_face<0, LSM>(F, bls, vbs, g, t.b, yy, zz)
+                                           : own_face<0, LSM>(F, g, vb.ie[0], gidx(g, t.x0 - 1, yy, zz), t.x0 - 1);
+                sW[q / TY][q % TY] = face_val<LSM>(f, dta);
+            }
         }
-        const double vS = yc == 1 ? vel_low_face<1, LSM>(F, bls, vbs, g, t.b, xc, z, dta)
-                                  : vel_face<1, LSM>(F, g, vb.ie[1], c - g.sy, yc - 1, dta);
-        const double fact = ((uE - uW) / dx + (vN - vS) / dy) + (wT - wB) / dz;
-        const double vol = ((fact * dx) * dy) * dz;
-        const double su = LSM ? vol / dt : fact / dt;
-        if (act) {
-            if (xc <= vb.ie[0]) F.u[c] = uE;
-            if (yc <= vb.ie[1]) F.v[c] = vN;
-            if (z <= vb.ie[2]) F.w[c] = wT;
-            if (rhs) rhs[c] = (TR)su;
-            if (su64) su64[c] = su;
-            rm = fmax(rm, fabs(vol));
+        // south halo: 64 columns x nz planes
+        for (int q = tid; q < SW_TX * nz; q += NT) {
+            const int xx = t.x0 + q % SW_TX, zz = t.z0 + q / SW_TX;
+            if (xx <= g.n0) {
+                const FaceOp f = t.y0 == 1 ? low_face<1, LSM>(F, bls, vbs, g, t.b, xx, zz)
+                                           : own_face<1, LSM>(F, g, vb.ie[1], gidx(g, xx, t.y0 - 1, zz), t.y0 - 1);
+                sS[q / SW_TX][q % SW_TX] = face_val<LSM>(f, dta);
+            }
         }
-        wB = wT;
+        const int x = t.x0 + tx, y = t.y0 + ty;
+        const bool act = x <= g.n0 && y <= g.n1;
+        // inactive lanes march a clamped column; what they compute goes to the sink
+        const int xc = act ? x : g.n0, yc = act ? y : g.n1;
+        const bool eown = xc == g.n0 || tx == SW_TX - 1;  // p(x+1) not in the lane to the right
+        const bool estored = xc > vb.ie[0], nstored = yc > vb.ie[1];
+        const double *esrc = estored ? F.u : F.us;
+        const double *nsrc = nstored ? F.v : F.vs;
+        double wB;  // bottom face of the first plane
+        {
+            const FaceOp f = t.z0 == 1 ? low_face<2, LSM>(F, bls, vbs, g, t.b, xc, yc)
+                                       : own_face<2, LSM>(F, g, vb.ie[2], gidx(g, xc, yc, t.z0 - 1), t.z0 - 1);
+            wB = face_val<LSM>(f, dta);
+        }
+        // operands of one plane; p and rho at the cell come from the plane below
+        struct Ops {
+            double us, vs, ws, pe, pn, pz, de, dn, dz;
+        };
+        auto load = [&](Ops &o, int z) {
+            const long long c = gidx(g, xc, yc, z);
+            const double *tsrc = z > vb.ie[2] ? F.w : F.ws;
+            o.us = esrc[c];
+            o.vs = nsrc[c];
+            o.ws = tsrc[c];
+            o.pe = F.p[c + 1];
+            o.pn = F.p[c + g.sy];
+            o.pz = F.p[c + g.sz];
+            if (LSM) {
+                o.de = F.dens[c + 1];
+                o.dn = F.dens[c + g.sy];
+                o.dz = F.dens[c + g.sz];
+            }
+        };
+        double pc, dc = 1.0;
+        {
+            const long long c = gidx(g, xc, yc, t.z0);
+            pc = F.p[c];
+            if (LSM) dc = F.dens[c];
+        }
+        auto step = [&](const Ops &o, int z) {
+            const long long c = gidx(g, xc, yc, z);
+            double pe = __shfl_down(pc, 1);
+            double de = LSM ? __shfl_down(dc, 1) : 1.0;
+            pe = eown ? o.pe : pe;
+            if (LSM) de = eown ? o.de : de;
+            const bool tstored = z > vb.ie[2];
+            const double uE = estored ? o.us : face_val<LSM>(o.us, pc, pe, dc, de, g.h0, dta);
+            const double vN = nstored ? o.vs : face_val<LSM>(o.vs, pc, o.pn, dc, o.dn, g.h1, dta);
+            const double wT = tstored ? o.ws : face_val<LSM>(o.ws, pc, o.pz, dc, o.dz, g.h2, dta);
+            double uW = __shfl_up(uE, 1);
+            uW = tx == 0 ? sW[z - t.z0][ty] : uW;
+            sN[z & 1][ty][tx] = vN;
+            __syncthreads();
+            const double vS = ty == 0 ? sS[z - t.z0][tx] : sN[z & 1][ty - 1][tx];
+            const double fact = ((uE - uW) / g.h0 + (vN - vS) / g.h1) + (wT - wB) / g.h2;
+            const double vol = ((fact * g.h0) * g.h1) * g.h2;
+            const double su = LSM ? vol / dt : fact / dt;
+            *((act && !estored) ? F.u + c : my_sink) = uE;
+            *((act && !nstored) ? F.v + c : my_sink) = vN;
+            *((act && !tstored) ? F.w + c : my_sink) = wT;
+            if (SU64) *(act ? su64 + c : my_sink) = su;
+            else *(act ? rhs + c : (TR *)my_sink) = (TR)su;
+            rm = act ? fmax(rm, fabs(vol)) : rm;
+            wB = wT;
+            pc = o.pz;
+            if (LSM) dc = o.dz;
+        };
+        Ops A, B;
+        load(A, t.z0);
+        __syncthreads();  // halos written
+        for (int z = t.z0;; z += 2) {
+            load(B, z + 1 <= t.z1 ? z + 1 : t.z1);
+            step(A, z);
+            if (z + 1 > t.z1) break;
+            load(A, z + 2 <= t.z1 ? z + 2 : t.z1);
+            step(B, z + 1);
+            if (z + 2 > t.z1) break;
+        }
     }
     if (!partials) return;
-    __shared__ double sm[SW_TX * TY];
-    const int tid = threadIdx.x + SW_TX * threadIdx.y;
+    static_assert(VEL_TZ_MAX * SW_TX >= NT, "the reduction reuses the south halo");
+    double *sm = &sS[0][0];
+    __syncthreads();  // the last tile's halo reads are done
     sm[tid] = rm;
     __syncthreads();
-    for (int w = SW_TX * TY / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (tid < w) sm[tid] = fmax(sm[tid], sm[tid + w]);
         __syncthreads();
     }

@@ -35,7 +35,7 @@ module mf3d_mg_f
   end type
 
   public :: mf3d_mg_create, mf3d_mg_set_density, mf3d_mg_vcycle, mf3d_mg_solve
-  public :: mf3d_mg_destroy, mf3d_mg_error
+  public :: mf3d_mg_newsolv, mf3d_mg_destroy, mf3d_mg_error
 
   interface
     integer(c_int) function mf3d_mg_create(cfg, blocks, out) bind(c, name='mf3d_mg_create')
@@ -61,6 +61,21 @@ module mf3d_mg_f
       import :: c_int, c_ptr, c_double, c_int32_t
       type(c_ptr), value :: mg
       type(c_ptr), intent(in) :: f(*), p(*)
+      real(c_double), intent(out) :: rmax
+      integer(c_int32_t), intent(out) :: iters
+      real(c_double), intent(out) :: hist(*)
+    end function
+
+    ! newsolv_mg in the velocity form: dom(ib)%ustar/%vstar/%wstar in,
+    ! %u/%v/%w/%p in and out, %su (or %sup) out (c_null_ptr entries or a
+    ! null array pointer skip it); eps, dt, alfapr as in control.cin
+    integer(c_int) function mf3d_mg_newsolv(mg, ustar, vstar, wstar, u, v, w, p, dt, alfapr, eps, su, &
+                                            rmax, iters, hist) bind(c, name='mf3d_mg_newsolv')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: mg
+      type(c_ptr), intent(in) :: ustar(*), vstar(*), wstar(*), u(*), v(*), w(*), p(*)
+      real(c_double), value :: dt, alfapr, eps
+      type(c_ptr), value :: su
       real(c_double), intent(out) :: rmax
       integer(c_int32_t), intent(out) :: iters
       real(c_double), intent(out) :: hist(*)

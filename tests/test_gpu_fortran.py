@@ -1,17 +1,19 @@
-"""The Fortran boundary end to end: a Fortran host (multiflow3d_amd/fortran)
-holding MultiFlow3D-style arrays with two ghost layers calls the library
-through the ISO_C_BINDING module; its pressure must equal the CPU oracle's
-bitwise."""
+"""The Fortran boundary end to end: Fortran hosts (multiflow3d_amd/fortran)
+holding MultiFlow3D-style arrays with two ghost layers call the library
+through the ISO_C_BINDING module; the Poisson-form pressure must equal the
+CPU oracle's bitwise, the velocity form the compiled reference's golden
+vectors."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from multiflow3d_amd.build import FORTRAN_HOST
+from multiflow3d_amd.build import FORTRAN_HOST, FORTRAN_HOST_VEL
 from oracle.oracle import OracleMG
 
 from .cases import case, inputs
+from .test_oracle_velocity import load_case
 
 pytestmark = pytest.mark.gpu
 
@@ -59,3 +61,48 @@ def test_fortran_host_matches_oracle(tmp_path, name):
     np.testing.assert_allclose(hist, rho_o, rtol=1e-12, atol=0)
     for a, b in zip(p_f, p_o):
         assert np.array_equal(a, b)
+
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_HOST_VEL), reason="Fortran host not built")
+@pytest.mark.parametrize("name", ["multi222_channel_L2", "mixedbc_L3", "lsm222_sphere_L2"])
+def test_fortran_velocity_host_matches_reference_fixture(tmp_path, name):
+    """A Fortran host with pl = 2 arrays calls mf3d_mg_newsolv: its p, u, v,
+    w (one-ghost-layer box), su and rmax history equal the compiled
+    reference's golden vectors, and the outer ghost layer is untouched."""
+    g, m, inp, exp = load_case(name)
+    fin = tmp_path / "in.bin"
+    parts = []
+    for b in range(len(g.blocks)):
+        parts += [inp["ustar"][b], inp["vstar"][b], inp["wstar"][b], inp["p0"][b]]
+        if m["lsm"]:
+            parts.append(inp["dens"][b])
+    np.concatenate([a.ravel() for a in parts]).tofile(fin)
+    out = tmp_path / "out.bin"
+    ctl = tmp_path / "ctl.txt"
+    nbx, nby, nbz = g.shape
+    ctl.write_text(f"{nbx} {nby} {nbz} {m['levels']} {int(m['lsm'])} {m['maxcy']}\n"
+                   f"{m['eps']!r} {m['dt']!r} {m['alfapr']!r}\n"
+                   + " ".join(map(str, g.bc)) + "\n" + " ".join(map(str, g.ex)) + "\n"
+                   + " ".join(map(str, g.ey)) + "\n" + " ".join(map(str, g.ez)) + "\n"
+                   + " ".join(repr(float(x)) for x in g.h) + f"\n{fin}\n{out}\n")
+    print(f"[fortran newsolv] {name}", flush=True)
+    res = subprocess.run([FORTRAN_HOST_VEL, str(ctl)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    k = int(raw[0])
+    assert k == exp["iters"]
+    assert np.array_equal(raw[1:k + 2], exp["rmax"])
+    o = k + 2
+    for b, blk in enumerate(g.blocks):
+        nx, ny, nz = blk.n
+        gsz = (nx + 2) * (ny + 2) * (nz + 2)
+        for key in ("p", "u", "v", "w"):
+            a = raw[o:o + gsz].reshape(nz + 2, ny + 2, nx + 2)
+            o += gsz
+            assert np.array_equal(a, exp[key][b]), (key, b)
+        isz = nx * ny * nz
+        assert np.array_equal(raw[o:o + isz].reshape(nz, ny, nx), exp["su"][b])
+        o += isz
+        assert raw[o] == 0.0, "the library wrote outside the one-cell ghost layer"
+        o += 1
