@@ -38,17 +38,19 @@ def algorithmic_bytes_per_cycle(n_cells_levels, w, c=0):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rbs_256cubed_L7_traffic.md")
+VEL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_velmas_256cubed_L7.md")
 
 
-def measured_traffic(form):
-    """HBM bytes per finest-level k_rbs launch of the 256^3, 7-level workload
-    (3 full sweeps + 1 zero-start sweep), from the committed rocprofv3 PMC
-    summary: FETCH_SIZE x 2 + WRITE_SIZE, the factors fixed by a 1 GiB copy
-    in the same runs.  None when the summary is absent."""
+def measured_traffic(form, path=TRAFFIC_FILE, key="traffic_bytes_per_launch:"):
+    """HBM bytes per launch of the 256^3, 7-level workload from a committed
+    rocprofv3 PMC summary: FETCH_SIZE x 2 + WRITE_SIZE, the factors fixed by
+    a 1 GiB copy in the same runs.  For k_rbs (default) the finest-level
+    launch mix (3 full sweeps + 1 zero-start sweep); for k_velmas the key
+    velocity_traffic_bytes_per_launch of its summary.  None when absent."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     kv = dict(item.split("=") for item in line.split(":", 1)[1].split())
                     return float(kv[form])
     except (OSError, ValueError, KeyError):
@@ -310,6 +312,9 @@ def main():
                              "bytes_per_launch": int(per_launch),
                              "achieved": round(per_launch / avg_s / 1e9, 1), "peak": HBM_PEAK_GBPS,
                              "unit": "GB/s", "frac": round(per_launch / avg_s / 1e9 / HBM_PEAK_GBPS, 4),
+                             "traffic": measured_traffic(prec, VEL_TRAFFIC_FILE,
+                                                         "velocity_traffic_bytes_per_launch:")
+                             if (n, L) == (256, 7) else None,
                              "model": "reads u*, v*, w*, p (32 B), writes u, v, w (24 B) and the "
                                       "V-cycle RHS (w B) per cell"}}
             vel[prec + "_u"] = v["u"]
