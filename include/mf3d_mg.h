@@ -3,6 +3,9 @@
  * drops in for MultiFlow3D's FP64 solver path:
  *
  *   subroutine coeff       mgsolver.for:2-222       -> mf3d_mg_create / mf3d_mg_set_density
+ *                                                     (and mf3d_mg_set_density_device)
+ *   subroutine heaviside   lsm.for:1456-1509        -> mf3d_mg_heaviside(_device), which
+ *                                                     rebuilds coeff's hierarchy as well
  *   subroutine mgkcyc      mgsolver.for:224-369     -> mf3d_mg_vcycle
  *   subroutine newsolv_mg  newsolv_mg.for:2-102     -> mf3d_mg_newsolv (the reference's own
  *                                                       velocity form, with calvel, calmas,
@@ -36,7 +39,7 @@
 extern "C" {
 #endif
 
-#define MF3D_MG_ABI_VERSION 3
+#define MF3D_MG_ABI_VERSION 4
 
 enum {
     MF3D_OK = 0,
@@ -105,6 +108,43 @@ int mf3d_mg_create(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, mf3d_m
 /* coeff, L_LSM branch (mgsolver.for:69-181): cell density dom(ib)%dens of
  * every block, caller layout.  Builds the whole level hierarchy on device. */
 int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks);
+
+/* The same from a device-resident density: every block's ghosted array
+ * ((n0+2)(n1+2)(n2+2) values, i fastest) concatenated in block order,
+ * mf3d_mg_ghosted_cells values (the layout of mf3d_mg_newsolv_device);
+ * the ghost layer is kept for the velocity form.  Runs on `stream` (NULL =
+ * the handle's own) and returns when the hierarchy is built. */
+int mf3d_mg_set_density_device(mf3d_mg *mg, const double *dens, void *stream);
+
+/* heaviside (lsm.for:1456-1509) followed by coeff (mgsolver.for:69-181),
+ * the density update of the reference's LSM time step (flosol.for:114-117)
+ * and the coefficient rebuild of newsolv_mg.for:12-14, without leaving
+ * the device.  Per block epsl = 2*max(dx, dy, dz); over every cell of the
+ * one-layer ghosted box
+ *   h = 0 (phi < -epsl), 1 (phi > epsl),
+ *       0.5*(1 + phi/epsl + 1/pi*sin(pi*phi/epsl)) otherwise, pi = 3.14159265359
+ *   dens = densg + (densl-densg)*h,  mu = mug + (mul-mug)*h,
+ * then bound_lsm(18/19) (bounds_lsm.for:2-104: zero-gradient ghosts on the
+ * physical non-periodic faces, E, W, B, T, S, N, the north loop also
+ * rewriting the last interior row from the one below it) and
+ * exchange(18/19).  With a periodic face the reference's exchange_bc
+ * selects no array for the density (exchange_bc.for:105,422); the path it
+ * defines for the viscosity (op 19) is applied to both.  phi must hold its
+ * ghost layer as the reference's bound_lsm(14) leaves it.  The handle's
+ * level-0 density (with its ghost layer) and coefficient hierarchy are
+ * replaced, as by mf3d_mg_set_density.  Needs variable_coef.
+ *
+ * Device form: phi, dens and mu are ghosted device arrays in the layout of
+ * mf3d_mg_newsolv_device; dens and mu may be NULL (not written).  Runs on
+ * `stream` (NULL = the handle's own) and returns when the hierarchy is
+ * built.  Host form: caller arrays (ghost >= 1); dens and mu (may be NULL)
+ * are written over the interior and the first ghost layer.  sin is the
+ * device library's: in the transition band |phi| <= epsl a value may
+ * differ from the reference's (host libm) in the last place. */
+int mf3d_mg_heaviside_device(mf3d_mg *mg, const double *phi, double densl, double densg, double mul,
+                             double mug, double *dens, double *mu, void *stream);
+int mf3d_mg_heaviside(mf3d_mg *mg, const double *const *phi_blocks, double densl, double densg,
+                      double mul, double mug, double *const *dens_blocks, double *const *mu_blocks);
 
 /* mgkcyc (mgsolver.for:224-369): one V-cycle on rhs (su, or sup under LSM);
  * pp is read and written including one ghost layer around the interior,
@@ -193,7 +233,14 @@ int64_t mf3d_mg_estimate_bytes(const mf3d_mg_cfg *cfg, const mf3d_block_desc *bl
  * counted.  mf3d_mg_time_kernel selects the kernel (MF3D_K_*; 0 = off)
  * and resets the counters; mf3d_mg_kernel_time returns the launches seen
  * and their summed time. */
-enum { MF3D_K_OFF = 0, MF3D_K_SMOOTH = 1, MF3D_K_OUTER = 2, MF3D_K_RESTRICT = 3, MF3D_K_PROLONG = 4 };
+enum {
+    MF3D_K_OFF = 0,
+    MF3D_K_SMOOTH = 1,
+    MF3D_K_OUTER = 2,
+    MF3D_K_RESTRICT = 3,
+    MF3D_K_PROLONG = 4,
+    MF3D_K_HEAVISIDE = 5
+};
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel);
 int mf3d_mg_kernel_time(mf3d_mg *mg, int64_t *launches, double *total_ms);
 

@@ -22,11 +22,12 @@ EXPORTS = (
     "mf3d_mg_estimate_bytes", "mf3d_mg_last_error", "mf3d_mg_destroy",
     "mf3d_mg_abi_version", "mf3d_mg_time_kernel", "mf3d_mg_kernel_time",
     "mf3d_mg_graph_replay", "mf3d_mg_set_eps", "mf3d_mg_newsolv", "mf3d_mg_newsolv_device",
-    "mf3d_mg_ghosted_cells",
+    "mf3d_mg_ghosted_cells", "mf3d_mg_set_density_device", "mf3d_mg_heaviside",
+    "mf3d_mg_heaviside_device",
 )
 MF3D_COARSE_REFERENCE, MF3D_COARSE_FACE = 0, 1
 MF3D_OUTER_DEFECT, MF3D_OUTER_GCR = 0, 1
-MF3D_K_OFF, MF3D_K_SMOOTH, MF3D_K_OUTER, MF3D_K_RESTRICT, MF3D_K_PROLONG = 0, 1, 2, 3, 4
+MF3D_K_OFF, MF3D_K_SMOOTH, MF3D_K_OUTER, MF3D_K_RESTRICT, MF3D_K_PROLONG, MF3D_K_HEAVISIDE = 0, 1, 2, 3, 4, 5
 
 
 class BlockDesc(ctypes.Structure):
@@ -103,5 +104,11 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int32), dp]
     L.mf3d_mg_ghosted_cells.restype = ctypes.c_int64
     L.mf3d_mg_ghosted_cells.argtypes = [vp]
+    L.mf3d_mg_set_density_device.restype = ctypes.c_int
+    L.mf3d_mg_set_density_device.argtypes = [vp, vp, vp]
+    L.mf3d_mg_heaviside.restype = ctypes.c_int
+    L.mf3d_mg_heaviside.argtypes = [vp, dpp, d, d, d, d, dpp, dpp]
+    L.mf3d_mg_heaviside_device.restype = ctypes.c_int
+    L.mf3d_mg_heaviside_device.argtypes = [vp, vp, d, d, d, d, vp, vp, vp]
     _lib = L
     return L

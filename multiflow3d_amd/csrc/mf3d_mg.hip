@@ -36,6 +36,7 @@
 #include "mg_sweep.hpp"
 #include "mg_tail.hpp"
 #include "mg_transfer.hpp"
+#include "mg_levelset.hpp"
 #include "mg_velocity.hpp"
 
 using namespace mf3d;
@@ -136,6 +137,15 @@ struct mf3d_mg {
     int n_vel = 0;
     hipGraphExec_t vgraph = nullptr;
     const void *vkey[7] = {};
+    // level-set density (heaviside), built at its first call: the composed
+    // map of bound_lsm(18) + exchange(18), the per-block epsl table, and
+    // the arenas of the host-array call (phi, mu)
+    bool ls_ready = false;
+    uint2 *d_lsmap = nullptr;
+    long long n_lsmap = 0;
+    LsBlock *d_lsb = nullptr;
+    long long ls_max_cells = 0;
+    double *d_lsio[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1171,10 +1181,12 @@ void drop_graphs(mf3d_mg *mg) {
 // Ghost entries of one field, per block: the arena offset of the entry
 // whose original value each ghost holds.  Slots: the x ghost planes
 // (whole), then the y ghost planes (interior i), then the z ghost planes
-// (interior i, j).
+// (interior i, j).  The few interior entries a sequence rewrites
+// (bound_lsm's north rows) are kept by arena offset in `inner`.
 struct GhostOrigins {
     const std::vector<BlockLevel> &hb;
     std::vector<std::vector<uint64_t>> o;
+    std::map<uint64_t, uint64_t> inner;
 
     explicit GhostOrigins(const std::vector<BlockLevel> &h) : hb(h), o(h.size()) {
         for (size_t b = 0; b < h.size(); ++b) {
@@ -1197,9 +1209,16 @@ struct GhostOrigins {
     }
     uint64_t get(int b, int i, int j, int k) const {
         const long long s = slot(b, i, j, k);
-        return s < 0 ? self(hb[b], i, j, k) : o[b][s];
+        if (s >= 0) return o[b][s];
+        const uint64_t me = self(hb[b], i, j, k);
+        const auto it = inner.find(me);
+        return it == inner.end() ? me : it->second;
     }
-    void set(int b, int i, int j, int k, uint64_t v) { o[b][slot(b, i, j, k)] = v; }
+    void set(int b, int i, int j, int k, uint64_t v) {
+        const long long s = slot(b, i, j, k);
+        if (s >= 0) o[b][s] = v;
+        else inner[self(hb[b], i, j, k)] = v;
+    }
     template <typename Fn>
     void for_ghosts(int b, Fn fn) const {
         const BlockLevel &g = hb[b];
@@ -1403,7 +1422,37 @@ void sym_pbound(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
     }
 }
 
-// Map entries {destination, source} of the ghosts the sequence changed.
+// bound_lsm (bounds_lsm.for:30-97) with one ghost layer: on each physical
+// non-periodic face, in the order E, W, B, T, S, N, the whole ghosted
+// ghost plane takes the plane inside it; the north loop starts at ipl = 0
+// (bounds_lsm.for:91), so the last interior row first takes row n-1.
+void sym_bound_lsm(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
+    constexpr int order[6] = {1, 0, 4, 5, 2, 3};
+    for (int b = 0; b < (int)blk.size(); ++b) {
+        const BlockLevel &g = S.hb[b];
+        for (const int f : order) {
+            if (!is_neumann(blk[b], f)) continue;
+            const int a = f / 2, n = ext_of(g, a);
+            const int e1 = ext_of(g, a == 0 ? 1 : 0) + 2, e2 = ext_of(g, a == 2 ? 1 : 2) + 2;
+            auto copy = [&](int dst, int src) {
+                for (int q = 0; q < e2; ++q)
+                    for (int p = 0; p < e1; ++p) {
+                        int d[3], s[3];
+                        plane_ijk(a, dst, p, q, d);
+                        plane_ijk(a, src, p, q, s);
+                        S.set(b, d[0], d[1], d[2], S.get(b, s[0], s[1], s[2]));
+                    }
+            };
+            if (f % 2 == 0) copy(0, 1);
+            else {
+                if (f == 3) copy(n, n - 1);
+                copy(n + 1, n);
+            }
+        }
+    }
+}
+
+// Map entries {destination, source} of the entries the sequence changed.
 void map_entries(const GhostOrigins &S, std::vector<uint2> &out) {
     for (int b = 0; b < (int)S.hb.size(); ++b) {
         const BlockLevel &g = S.hb[b];
@@ -1412,6 +1461,8 @@ void map_entries(const GhostOrigins &S, std::vector<uint2> &out) {
             if (org != self) out.push_back(make_uint2((uint32_t)self, (uint32_t)org));
         });
     }
+    for (const auto &e : S.inner)
+        if (e.first != e.second) out.push_back(make_uint2((uint32_t)e.first, (uint32_t)e.second));
 }
 
 int build_vel(mf3d_mg *mg) {
@@ -1607,6 +1658,70 @@ int newsolv_any(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, doubl
                : newsolv_dev<double, false>(mg, F, dt, alfapr, eps, su, rmax_out, iters_out, hist);
 }
 
+// ------------------------------------------------ level-set density (host)
+
+// The map of bound_lsm(18) + exchange(18) over the level-0 arena and the
+// per-block epsl table (lsm.for:1475).
+int build_ls(mf3d_mg *mg) {
+    if (mg->ls_ready) return 0;
+    const Level &L0 = mg->lv[0];
+    const int nb = mg->cfg.nblocks;
+    if (L0.cells_gh >= (1LL << 32)) {
+        mg->err = "heaviside: a level-0 field holds 2^32 entries or more";
+        return MF3D_ENOTSUP;
+    }
+    const bool periodic = std::any_of(mg->blk.begin(), mg->blk.end(), [](const mf3d_block_desc &B) {
+        return std::any_of(B.bc, B.bc + 6, [](int c) { return c == 5; });
+    });
+    GhostOrigins S(L0.hb);
+    sym_bound_lsm(mg->blk, S);
+    if (periodic) sym_exchange_bc(mg->blk, S, -1);
+    if (!sym_exchange_smlvl(mg->blk, S)) {
+        mg->err = "heaviside: edge or corner neighbours are ambiguous (block grid not conforming)";
+        return MF3D_ENOTSUP;
+    }
+    std::vector<uint2> ent;
+    map_entries(S, ent);
+    std::vector<LsBlock> lb(nb);
+    long long mx = 0;
+    for (int b = 0; b < nb; ++b) {
+        const BlockLevel &g = L0.hb[b];
+        const double *h = mg->blk[b].h;
+        lb[b].off = g.off;
+        lb[b].cells = (long long)(g.n0 + 2) * (g.n1 + 2) * (g.n2 + 2);
+        lb[b].epsl = 2 * std::max(std::max(h[0], h[1]), h[2]);
+        mx = std::max(mx, lb[b].cells);
+    }
+    int rc;
+    if ((rc = upload(mg, &mg->d_lsb, lb)) || (!ent.empty() && (rc = upload(mg, &mg->d_lsmap, ent)))) return rc;
+    mg->n_lsmap = (long long)ent.size();
+    mg->ls_max_cells = mx;
+    mg->ls_ready = true;
+    return 0;
+}
+
+// heaviside + bound_lsm(18/19) into the level-0 density (and the caller's
+// copies), then coeff from it.
+int heaviside_dev(mf3d_mg *mg, const double *phi, const LsProps &pr, double *dens_out, double *mu) {
+    int rc = build_ls(mg);
+    if (rc) return rc;
+    Level &L0 = mg->lv[0];
+    const int nb = mg->cfg.nblocks;
+    const bool tk = tk_on(mg, MF3D_K_HEAVISIDE, 0);
+    if (tk) tk_mark(mg);
+    hipLaunchKernelGGL(k_heaviside, dim3((unsigned)((mg->ls_max_cells + 1023) / 1024), (unsigned)nb), dim3(256), 0,
+                       mg->stream, phi, L0.dens, dens_out, mu, pr, mg->d_lsb);
+    if (tk) tk_mark(mg);
+    if (mg->n_lsmap > 0) {
+        const unsigned gb = (unsigned)std::min<long long>((mg->n_lsmap + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_heaviside_map, dim3(gb), dim3(256), 0, mg->stream, phi, mg->d_lsmap, mg->n_lsmap,
+                           L0.dens, dens_out, mu, pr, mg->d_lsb, nb);
+    }
+    HIPCHK(hipGetLastError());
+    mg->dens_ghosted = true;
+    return build_lsm(mg);
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -1670,6 +1785,82 @@ int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks) {
                            L0.db, L0.t_in);
     }
     return build_lsm(mg);
+}
+
+int mf3d_mg_set_density_device(mf3d_mg *mg, const double *dens, void *stream) {
+    if (!mg || !dens) return MF3D_EINVAL;
+    if (!mg->cfg.variable_coef) {
+        mg->err = "handle was created without variable_coef";
+        return MF3D_EINVAL;
+    }
+    int rc = set_device(mg);
+    if (rc) return rc;
+    Level &L0 = mg->lv[0];
+    hipStream_t own = mg->stream;
+    if (stream) mg->stream = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(L0.dens, dens, sizeof(double) * L0.cells_gh, hipMemcpyDeviceToDevice, mg->stream);
+    if (e == hipSuccess) {
+        mg->dens_ghosted = true;
+        rc = build_lsm(mg);
+    } else {
+        mg->err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e);
+        rc = MF3D_EDEVICE;
+    }
+    mg->stream = own;
+    return rc;
+}
+
+int mf3d_mg_heaviside_device(mf3d_mg *mg, const double *phi, double densl, double densg, double mul, double mug,
+                             double *dens, double *mu, void *stream) {
+    if (!mg || !phi) return MF3D_EINVAL;
+    if (!mg->cfg.variable_coef) {
+        mg->err = "handle was created without variable_coef";
+        return MF3D_EINVAL;
+    }
+    int rc = set_device(mg);
+    if (rc) return rc;
+    hipStream_t own = mg->stream;
+    if (stream) mg->stream = (hipStream_t)stream;
+    rc = heaviside_dev(mg, phi, LsProps{densl, densg, mul, mug}, dens, mu);
+    mg->stream = own;
+    return rc;
+}
+
+int mf3d_mg_heaviside(mf3d_mg *mg, const double *const *phi_blocks, double densl, double densg, double mul,
+                      double mug, double *const *dens_blocks, double *const *mu_blocks) {
+    if (!mg || !phi_blocks) return MF3D_EINVAL;
+    if (!mg->cfg.variable_coef) {
+        mg->err = "handle was created without variable_coef";
+        return MF3D_EINVAL;
+    }
+    for (int b = 0; b < mg->cfg.nblocks; ++b)
+        if (mg->blk[b].ghost < 1) {
+            mg->err = "heaviside needs caller arrays with ghost >= 1";
+            return MF3D_EINVAL;
+        }
+    int rc = set_device(mg);
+    if (rc) return rc;
+    Level &L0 = mg->lv[0];
+    const size_t bytes = sizeof(double) * (size_t)L0.cells_gh;
+    for (int i = 0; i < 2; ++i)
+        if (!mg->d_lsio[i] && (i == 0 || mu_blocks) && (rc = dev_alloc(mg, (void **)&mg->d_lsio[i], bytes)))
+            return rc;
+    host_to_packed(mg, phi_blocks, mg->h_io, 1);
+    HIPCHK(hipMemcpyAsync(mg->d_lsio[0], mg->h_io, bytes, hipMemcpyHostToDevice, mg->stream));
+    HIPCHK(hipStreamSynchronize(mg->stream));
+    if ((rc = heaviside_dev(mg, mg->d_lsio[0], LsProps{densl, densg, mul, mug}, nullptr,
+                            mu_blocks ? mg->d_lsio[1] : nullptr)))
+        return rc;
+    const double *src[2] = {L0.dens, mg->d_lsio[1]};
+    double *const *dst[2] = {dens_blocks, mu_blocks};
+    for (int i = 0; i < 2; ++i) {
+        if (!dst[i]) continue;
+        HIPCHK(hipMemcpyAsync(mg->h_io, src[i], bytes, hipMemcpyDeviceToHost, mg->stream));
+        HIPCHK(hipStreamSynchronize(mg->stream));
+        packed_to_host(mg, mg->h_io, dst[i], 1);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 int mf3d_mg_vcycle(mf3d_mg *mg, const double *const *rhs_blocks, double *const *pp_blocks) {
@@ -1803,7 +1994,7 @@ int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const
 int64_t mf3d_mg_ghosted_cells(const mf3d_mg *mg) { return mg ? mg->lv[0].cells_gh : MF3D_EINVAL; }
 
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel) {
-    if (!mg || kernel < MF3D_K_OFF || kernel > MF3D_K_PROLONG) return MF3D_EINVAL;
+    if (!mg || kernel < MF3D_K_OFF || kernel > MF3D_K_HEAVISIDE) return MF3D_EINVAL;
     if (mg->stream) (void)hipStreamSynchronize(mg->stream);
     tk_flush(mg);
     if (kernel != mg->tk) drop_timed_graphs(mg);

@@ -3,23 +3,28 @@
 !
 ! It holds the fields the way MultiFlow3D does -- per block, column-major
 ! arrays with pl = 2 ghost layers (initial.for:199-204): ustar, vstar,
-! wstar, u, v, w, p, su, dens -- and replaces the whole newsolv_mg call
-! (newsolv_mg.for:2-102) with one mf3d_mg_newsolv through the ISO_C_BINDING
-! module, passing each block's arrays by address.  The outer ghost layer
-! is filled with a marker the library must leave alone.
+! wstar, u, v, w, p, su, dens (phi, mu) -- and replaces the whole
+! newsolv_mg call (newsolv_mg.for:2-102) with one mf3d_mg_newsolv through
+! the ISO_C_BINDING module, passing each block's arrays by address.  With
+! lsm = 2 the density comes from a level set first, as the reference's
+! step does (flosol.for:114-117): mf3d_mg_heaviside replaces heaviside
+! (lsm.for:1456-1509) and the coeff of newsolv_mg.for:12-14.  The outer
+! ghost layer is filled with a marker the library must leave alone.
 !
 ! Usage: mf3d_host_vel <control file>
 !   nbx nby nbz levels lsm maxcy
 !   eps dt alfapr
+!   (lsm = 2 only) densl densg mul mug
 !   bc(6)
 !   ex(nbx) / ey(nby) / ez(nbz)
 !   dx dy dz
-!   input file: per block ustar, vstar, wstar, p (and dens under LSM), each
-!     the block's one-ghost-layer box (n+2)^3, f64, i fastest
+!   input file: per block ustar, vstar, wstar, p (and dens under lsm = 1,
+!     phi under lsm = 2), each the block's one-ghost-layer box (n+2)^3,
+!     f64, i fastest
 !   output file
 ! Output (f64 stream): iters, rmax(0:iters), then per block the one-layer
-! boxes of p, u, v, w, the interior of su, and the count of outer-layer
-! entries that changed.
+! boxes of p, u, v, w, the interior of su, (lsm = 2) the one-layer boxes
+! of dens and mu, and the count of outer-layer entries that changed.
 program mf3d_host_vel
   use iso_c_binding
   use mf3d_mg_f
@@ -29,15 +34,17 @@ program mf3d_host_vel
   type blockfield
     real(c_double), allocatable :: ustar(:,:,:), vstar(:,:,:), wstar(:,:,:)
     real(c_double), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:), p(:,:,:), su(:,:,:), dens(:,:,:)
+    real(c_double), allocatable :: phi(:,:,:), mu(:,:,:)
   end type
   type(blockfield), allocatable, target :: dom(:)
   type(mf3d_block_desc), allocatable :: desc(:)
   type(mf3d_mg_cfg) :: cfg
   type(c_ptr) :: mg
   type(c_ptr), allocatable, target :: pu(:), pv(:), pw(:), pus(:), pvs(:), pws(:), pp(:), psu(:), pd(:)
+  type(c_ptr), allocatable, target :: pphi(:), pmu(:)
   integer :: nbx, nby, nbz, lsmf, maxcy, levels, bcs(6), a, b, c, ib, nb, u, rc, n(3)
   integer, allocatable :: ex(:), ey(:), ez(:)
-  real(c_double) :: eps, dt, alfapr, h(3), rmax
+  real(c_double) :: eps, dt, alfapr, h(3), rmax, densl, densg, mul, mug
   real(c_double), allocatable :: hist(:)
   integer(c_int32_t) :: iters
   integer(8) :: changed
@@ -47,6 +54,7 @@ program mf3d_host_vel
   open(newunit=u, file=trim(ctl), status='old')
   read(u, *) nbx, nby, nbz, levels, lsmf, maxcy
   read(u, *) eps, dt, alfapr
+  if (lsmf == 2) read(u, *) densl, densg, mul, mug
   read(u, *) bcs
   allocate(ex(nbx), ey(nby), ez(nbz))
   read(u, *) ex
@@ -59,7 +67,7 @@ program mf3d_host_vel
 
   nb = nbx * nby * nbz
   allocate(dom(nb), desc(nb), hist(maxcy + 1))
-  allocate(pu(nb), pv(nb), pw(nb), pus(nb), pvs(nb), pws(nb), pp(nb), psu(nb), pd(nb))
+  allocate(pu(nb), pv(nb), pw(nb), pus(nb), pvs(nb), pws(nb), pp(nb), psu(nb), pd(nb), pphi(nb), pmu(nb))
   do c = 0, nbz - 1
     do b = 0, nby - 1
       do a = 0, nbx - 1
@@ -68,6 +76,7 @@ program mf3d_host_vel
         call alloc3(dom(ib)%ustar, n); call alloc3(dom(ib)%vstar, n); call alloc3(dom(ib)%wstar, n)
         call alloc3(dom(ib)%u, n); call alloc3(dom(ib)%v, n); call alloc3(dom(ib)%w, n)
         call alloc3(dom(ib)%p, n); call alloc3(dom(ib)%su, n); call alloc3(dom(ib)%dens, n)
+        call alloc3(dom(ib)%phi, n); call alloc3(dom(ib)%mu, n)
         desc(ib)%n = n
         desc(ib)%ghost = pl
         desc(ib)%ld = n + 2*pl
@@ -92,6 +101,7 @@ program mf3d_host_vel
     call read_box(u, dom(ib)%wstar)
     call read_box(u, dom(ib)%p)
     if (lsmf == 1) call read_box(u, dom(ib)%dens)
+    if (lsmf == 2) call read_box(u, dom(ib)%phi)
     ! u, v, w start as the predicted velocities, as in the reference's step
     dom(ib)%u = dom(ib)%ustar
     dom(ib)%v = dom(ib)%vstar
@@ -108,7 +118,7 @@ program mf3d_host_vel
   cfg%eps = eps
   cfg%min_cycles = 2
   cfg%precision = MF3D_FP64
-  cfg%variable_coef = lsmf
+  cfg%variable_coef = merge(1, 0, lsmf >= 1)
   cfg%device = 0
   rc = mf3d_mg_create(cfg, desc, mg)
   if (rc /= MF3D_OK) then
@@ -119,7 +129,15 @@ program mf3d_host_vel
     pus(ib) = c_loc(dom(ib)%ustar); pvs(ib) = c_loc(dom(ib)%vstar); pws(ib) = c_loc(dom(ib)%wstar)
     pu(ib) = c_loc(dom(ib)%u); pv(ib) = c_loc(dom(ib)%v); pw(ib) = c_loc(dom(ib)%w)
     pp(ib) = c_loc(dom(ib)%p); psu(ib) = c_loc(dom(ib)%su); pd(ib) = c_loc(dom(ib)%dens)
+    pphi(ib) = c_loc(dom(ib)%phi); pmu(ib) = c_loc(dom(ib)%mu)
   end do
+  if (lsmf == 2) then   ! flosol.for:116 heaviside, then newsolv_mg.for:12-14 coeff
+    rc = mf3d_mg_heaviside(mg, pphi, densl, densg, mul, mug, c_loc(pd), c_loc(pmu))
+    if (rc /= MF3D_OK) then
+      print *, 'heaviside failed ', rc, ' ', mf3d_mg_error(mg)
+      stop 1
+    end if
+  end if
   if (lsmf == 1) then   ! newsolv_mg.for:12-14: coeff every call under LSM
     rc = mf3d_mg_set_density(mg, pd)
     if (rc /= MF3D_OK) then
@@ -150,6 +168,11 @@ program mf3d_host_vel
     call write_box(u, dom(ib)%su, 0)
     changed = outer_changed(dom(ib)%p) + outer_changed(dom(ib)%u) + outer_changed(dom(ib)%v) &
               + outer_changed(dom(ib)%w)
+    if (lsmf == 2) then
+      call write_box(u, dom(ib)%dens, 1)
+      call write_box(u, dom(ib)%mu, 1)
+      changed = changed + outer_changed(dom(ib)%dens) + outer_changed(dom(ib)%mu)
+    end if
     write(u) dble(changed)
   end do
   close(u)

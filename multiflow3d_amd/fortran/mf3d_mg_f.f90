@@ -35,7 +35,7 @@ module mf3d_mg_f
   end type
 
   public :: mf3d_mg_create, mf3d_mg_set_density, mf3d_mg_vcycle, mf3d_mg_solve
-  public :: mf3d_mg_newsolv, mf3d_mg_destroy, mf3d_mg_error
+  public :: mf3d_mg_newsolv, mf3d_mg_heaviside, mf3d_mg_destroy, mf3d_mg_error
 
   interface
     integer(c_int) function mf3d_mg_create(cfg, blocks, out) bind(c, name='mf3d_mg_create')
@@ -79,6 +79,20 @@ module mf3d_mg_f
       real(c_double), intent(out) :: rmax
       integer(c_int32_t), intent(out) :: iters
       real(c_double), intent(out) :: hist(*)
+    end function
+
+    ! heaviside (lsm.for:1456-1509) + bound_lsm(18/19), then coeff from the
+    ! new density: dom(ib)%phi in; dom(ib)%dens and %mu out over the
+    ! interior and the first ghost layer (c_loc of an array of their
+    ! addresses, or c_null_ptr to skip); densl, densg, mul, mug as in
+    ! control.cin
+    integer(c_int) function mf3d_mg_heaviside(mg, phi, densl, densg, mul, mug, dens, mu) &
+        bind(c, name='mf3d_mg_heaviside')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: mg
+      type(c_ptr), intent(in) :: phi(*)
+      real(c_double), value :: densl, densg, mul, mug
+      type(c_ptr), value :: dens, mu
     end function
 
     subroutine mf3d_mg_destroy(mg) bind(c, name='mf3d_mg_destroy')

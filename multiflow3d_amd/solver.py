@@ -182,6 +182,39 @@ class PressureMultigrid:
         if rc != 0:
             raise MGError(f"coeff failed ({rc}): {self._err()}")
 
+    def coeff_device(self, dens_ptr: int, stream_ptr: int = 0):
+        """coeff from a device-resident density: ghosted_cells doubles (all
+        blocks' ghosted arrays, the layout of newsolv_device)."""
+        rc = _lib.lib().mf3d_mg_set_density_device(self._h, ctypes.c_void_p(int(dens_ptr)),
+                                                   ctypes.c_void_p(stream_ptr or None))
+        if rc != 0:
+            raise MGError(f"coeff failed ({rc}): {self._err()}")
+
+    def heaviside(self, phi, densl: float, densg: float, mul: float, mug: float):
+        """heaviside (lsm.for:1456-1509) + bound_lsm(18/19), then coeff from
+        the new density.  phi: per-block ghosted arrays (nz+2, ny+2, nx+2)
+        with their ghost layer filled.  Returns (dens, mu) as new ghosted
+        arrays (interior and first ghost layer)."""
+        ph = self._vel_arrays(phi, "phi", False)
+        dens = [np.zeros_like(a) for a in ph]
+        mu = [np.zeros_like(a) for a in ph]
+        rc = _lib.lib().mf3d_mg_heaviside(self._h, self._ptrs(ph), float(densl), float(densg), float(mul),
+                                          float(mug), self._ptrs(dens), self._ptrs(mu))
+        if rc != 0:
+            raise MGError(f"heaviside failed ({rc}): {self._err()}")
+        return dens, mu
+
+    def heaviside_device(self, phi_ptr: int, densl: float, densg: float, mul: float, mug: float,
+                         dens_ptr: int = 0, mu_ptr: int = 0, stream_ptr: int = 0):
+        """Device-resident heaviside + coeff: phi_ptr, dens_ptr, mu_ptr are
+        ghosted_cells doubles each (dens and mu optional)."""
+        rc = _lib.lib().mf3d_mg_heaviside_device(
+            self._h, ctypes.c_void_p(int(phi_ptr)), float(densl), float(densg), float(mul), float(mug),
+            ctypes.c_void_p(dens_ptr or None), ctypes.c_void_p(mu_ptr or None),
+            ctypes.c_void_p(stream_ptr or None))
+        if rc != 0:
+            raise MGError(f"heaviside failed ({rc}): {self._err()}")
+
     def mgkcyc(self, rhs, pp):
         """One V-cycle on ``rhs``; ``pp`` (ghosted arrays) is updated in place and returned."""
         r = self._ghosted(rhs)

@@ -248,26 +248,20 @@ def have_ref_vel() -> bool:
     return os.path.exists(REF_VEL_BIN)
 
 
-def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: int, dt: float,
-                alfapr: float, lsm: bool = False, dens=None, nu_pre: int = 2, nu_post: int = 2):
-    """Run the compiled reference's velocity-form newsolv_mg loop (calvel,
-    calmas, mgkcyc, p update, exchange(4), pbound).  Every field is a list of
-    full ghosted block arrays (one ghost layer), shaped (z+2, y+2, x+2).
-    Returns (iterations, rmax history, p, u, v, w, su) with the fields as
-    lists of ghosted block arrays."""
+def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None, nu_pre=2, nu_post=2):
+    """One run of mg_ref_vel in `mode` (0 constant, 1 density, 2 level set,
+    3 bound_lsm only) on the concatenated input `parts`; returns its raw
+    f64 output."""
     if not have_ref_vel():
         raise RuntimeError("compiled reference not built (make -C oracle ref)")
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
-        parts = []
-        for b in range(len(grid.blocks)):
-            parts += [ustar[b], vstar[b], wstar[b], p0[b]] + ([dens[b]] if lsm else [])
         _concat(parts).tofile(fin)
         fout = os.path.join(td, "out.bin")
         ctl = os.path.join(td, "ctl.txt")
         with open(ctl, "w") as fh:
             nbx, nby, nbz = grid.shape
-            fh.write(f"{nbx} {nby} {nbz} {levels} {int(lsm)} {maxcy} {nu_pre} {nu_post}\n")
+            fh.write(f"{nbx} {nby} {nbz} {levels} {mode} {maxcy} {nu_pre} {nu_post}\n")
             fh.write(f"{eps!r}\n")
             fh.write(" ".join(str(c) for c in grid.bc) + "\n")
             fh.write(" ".join(map(str, grid.ex)) + "\n")
@@ -275,17 +269,55 @@ def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: i
             fh.write(" ".join(map(str, grid.ez)) + "\n")
             fh.write(" ".join(repr(float(x)) for x in grid.h) + "\n")
             fh.write(f"{dt!r} {alfapr!r}\n")
+            if mode == 2:
+                fh.write(" ".join(repr(float(x)) for x in props) + "\n")
             fh.write(fin + "\n" + fout + "\n")
         subprocess.run([REF_VEL_BIN, ctl], check=True, capture_output=True, timeout=600)
-        raw = np.fromfile(fout, dtype=np.float64)
-    k = int(raw[0])
-    o = 1
-    rmh = raw[o:o + maxcy + 1].copy(); o += maxcy + 1
-    out = {"p": [], "u": [], "v": [], "w": [], "su": []}
+        return np.fromfile(fout, dtype=np.float64)
+
+
+def _boxes(grid, raw, o, keys):
+    out = {key: [] for key in keys}
     for blk in grid.blocks:
         nx, ny, nz = blk.n
         m = (nx + 2) * (ny + 2) * (nz + 2)
-        for key in ("p", "u", "v", "w", "su"):
+        for key in keys:
             out[key].append(raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy())
             o += m
-    return k, rmh[:k + 1], out["p"], out["u"], out["v"], out["w"], out["su"]
+    return out, o
+
+
+def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: int, dt: float,
+                alfapr: float, lsm: bool = False, dens=None, nu_pre: int = 2, nu_post: int = 2,
+                phi=None, props=None):
+    """Run the compiled reference's velocity-form newsolv_mg loop (calvel,
+    calmas, mgkcyc, p update, exchange(4), pbound).  Every field is a list of
+    full ghosted block arrays (one ghost layer), shaped (z+2, y+2, x+2).
+    Returns (iterations, rmax history, p, u, v, w, su) with the fields as
+    lists of ghosted block arrays.  With a level set `phi` and
+    props = (densl, densg, mul, mug) the reference's heaviside runs first
+    (lsm.for:1456-1509) and dens and mu are returned as well."""
+    mode = 2 if phi is not None else int(lsm)
+    parts = []
+    for b in range(len(grid.blocks)):
+        parts += [ustar[b], vstar[b], wstar[b], p0[b]]
+        if mode == 1:
+            parts.append(dens[b])
+        elif mode == 2:
+            parts.append(phi[b])
+    raw = _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props, nu_pre, nu_post)
+    k = int(raw[0])
+    o = 1
+    rmh = raw[o:o + maxcy + 1].copy(); o += maxcy + 1
+    keys = ("p", "u", "v", "w", "su") + (("dens", "mu") if mode == 2 else ())
+    out, o = _boxes(grid, raw, o, keys)
+    res = (k, rmh[:k + 1], out["p"], out["u"], out["v"], out["w"], out["su"])
+    return res + (out["dens"], out["mu"]) if mode == 2 else res
+
+
+def run_ref_bound_lsm(grid, field):
+    """The compiled reference's bound_lsm(19) (bounds_lsm.for:2-104, with
+    exchange(19)) on a cell field given as ghosted block arrays; returns the
+    field after it."""
+    raw = _ref_vel_call(grid, 1, 3, list(field), 0, 0.0, 1.0, 1.0)
+    return _boxes(grid, raw, 0, ("f",))[0]["f"]

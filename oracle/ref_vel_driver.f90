@@ -8,6 +8,12 @@
 ! TEST INFRASTRUCTURE ONLY: it produces golden vectors in the build
 ! container.  All blocks live on one MPI rank.
 !
+! lsm = 2 runs heaviside (lsm.for:1456-1509; bound_lsm(18/19) and
+! exchange(18/19) after it, bounds_lsm.for:2-104) on a level set phi before
+! newsolv_mg, as the reference's time step does (flosol.for:114-117), and
+! writes dens and mu as well.  lsm = 3 only fills the ghosts of a given
+! cell field through bound_lsm(19) (the viscosity's path) and writes it.
+!
 ! Usage: mg_ref_vel <control file>
 !   line 1: nbx nby nbz levels lsm maxcy nu_pre nu_post
 !   line 2: eps (absolute: rmax = max|div u| dV, newsolv_mg.for:56)
@@ -15,11 +21,14 @@
 !   lines 4-6: block extents along x, y, z
 !   line 7: dx dy dz
 !   line 8: dt alfapr
+!   (lsm = 2 only) densl densg mul mug
 !   line 9: input file: per block, full ghosted arrays (pl = 1) of ustar,
-!           vstar, wstar, p and (lsm) dens, in that order
+!           vstar, wstar, p and dens (lsm = 1) or phi (lsm = 2), in that
+!           order; lsm = 3: per block the field alone
 !   line 10: output file
 ! Output (f64 stream): iterations, rmax(0:maxcy), then per block the full
-! ghosted arrays of p, u, v, w and su (sup under lsm).
+! ghosted arrays of p, u, v, w and su (sup under lsm), then (lsm = 2) dens
+! and mu.  lsm = 3: per block the field after bound_lsm(19).
 program mg_ref_vel_driver
   use vars
   use multidata
@@ -46,6 +55,7 @@ program mg_ref_vel_driver
   read(u, *) ez
   read(u, *) gdx, gdy, gdz
   read(u, *) dt, alfapr
+  if (lsmf == 2) read(u, *) densl, densg, mul, mug
   read(u, '(a)') fin
   read(u, '(a)') fout
   close(u)
@@ -54,7 +64,7 @@ program mg_ref_vel_driver
   nbp = num_domains
   ngrd_gl = ngrid_input
   mg_itrsch = 3
-  L_LSM = (lsmf == 1)
+  L_LSM = (lsmf == 1 .or. lsmf == 2)
   pl = 1
   pl_ex = 0
   differencing = 1
@@ -66,6 +76,11 @@ program mg_ref_vel_driver
   rdiv = 1
   rdv = 1
   PERIODIC = any(bcs == 5)
+  if (lsmf == 2 .and. PERIODIC) then
+    ! exchange_bc.for:105,422 select no array for op 18 in the pack loops
+    print *, 'heaviside with periodic faces is not run: exchange_bc(18) has no array'
+    stop 2
+  end if
   ngg = 0
 
   do ic2 = 0, nbz - 1
@@ -179,7 +194,30 @@ program mg_ref_vel_driver
     allocate(dom(ib)%dens_mg(max(tot, 1)))
     allocate(dom(ib)%dens(dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%ttc_k))
     dom(ib)%dens = 1.0
+    if (lsmf >= 2) then
+      allocate(dom(ib)%phi(dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%ttc_k), &
+               dom(ib)%h_phi(dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%ttc_k), &
+               dom(ib)%mu(dom(ib)%ttc_i, dom(ib)%ttc_j, dom(ib)%ttc_k))
+      dom(ib)%h_phi = 0.0
+      dom(ib)%mu = 0.0
+    end if
   end do
+
+  if (lsmf == 3) then
+    open(newunit=uu, file=trim(fin), access='stream', form='unformatted', status='old')
+    do ib = 1, nbp
+      read(uu) dom(ib)%mu
+    end do
+    close(uu)
+    call bound_lsm(19)
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    do ib = 1, nbp
+      write(u) dom(ib)%mu
+    end do
+    close(u)
+    call MPI_FINALIZE(ierr2)
+    stop
+  end if
 
   open(newunit=uu, file=trim(fin), access='stream', form='unformatted', status='old')
   do ib = 1, nbp
@@ -187,7 +225,8 @@ program mg_ref_vel_driver
     read(uu) dom(ib)%vstar
     read(uu) dom(ib)%wstar
     read(uu) dom(ib)%p
-    if (L_LSM) read(uu) dom(ib)%dens
+    if (lsmf == 1) read(uu) dom(ib)%dens
+    if (lsmf == 2) read(uu) dom(ib)%phi
     dom(ib)%u = dom(ib)%ustar
     dom(ib)%v = dom(ib)%vstar
     dom(ib)%w = dom(ib)%wstar
@@ -196,6 +235,7 @@ program mg_ref_vel_driver
 
   allocate(rmh(0:maxcy))
   rmh = 0.0
+  if (lsmf == 2) call heaviside
   ! newsolv_mg.for:12-21
   call coeff
   call calvel
@@ -241,6 +281,10 @@ program mg_ref_vel_driver
       write(u) dom(ib)%sup
     else
       write(u) dom(ib)%su
+    end if
+    if (lsmf == 2) then
+      write(u) dom(ib)%dens
+      write(u) dom(ib)%mu
     end if
   end do
   close(u)

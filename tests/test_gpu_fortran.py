@@ -106,3 +106,64 @@ def test_fortran_velocity_host_matches_reference_fixture(tmp_path, name):
         o += isz
         assert raw[o] == 0.0, "the library wrote outside the one-cell ghost layer"
         o += 1
+
+
+@pytest.mark.skipif(not os.path.exists(FORTRAN_HOST_VEL), reason="Fortran host not built")
+@pytest.mark.parametrize("name", ["walls221_band_L2", "walls222_sphere_L2"])
+def test_fortran_levelset_host_matches_reference_fixture(tmp_path, name):
+    """The reference's LSM step order from a Fortran host with pl = 2
+    arrays: mf3d_mg_heaviside on phi (dens, mu out), then mf3d_mg_newsolv.
+    dens and mu to the last-place bar of the level-set tests, the chain
+    bitwise where the densities are; the outer ghost layer untouched."""
+    from .test_gpu_levelset import _check_dens, _rel
+    from .test_oracle_levelset import load_ls
+    g, m, a = load_ls(name)
+    densl, densg, mul, mug = a["props"]
+    fin = tmp_path / "in.bin"
+    parts = []
+    for b in range(len(g.blocks)):
+        parts += [a["ustar"][b], a["vstar"][b], a["wstar"][b], a["p0"][b], a["phi"][b]]
+    np.concatenate([x.ravel() for x in parts]).tofile(fin)
+    out = tmp_path / "out.bin"
+    ctl = tmp_path / "ctl.txt"
+    nbx, nby, nbz = g.shape
+    ctl.write_text(f"{nbx} {nby} {nbz} {m['levels']} 2 {m['maxcy']}\n"
+                   f"0.0 {m['dt']!r} {m['alfapr']!r}\n"
+                   f"{densl!r} {densg!r} {mul!r} {mug!r}\n"
+                   + " ".join(map(str, g.bc)) + "\n" + " ".join(map(str, g.ex)) + "\n"
+                   + " ".join(map(str, g.ey)) + "\n" + " ".join(map(str, g.ez)) + "\n"
+                   + " ".join(repr(float(x)) for x in g.h) + f"\n{fin}\n{out}\n")
+    print(f"[fortran heaviside] {name}", flush=True)
+    res = subprocess.run([FORTRAN_HOST_VEL, str(ctl)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    k = int(raw[0])
+    assert k == a["iters"]
+    hist = raw[1:k + 2]
+    o = k + 2
+    got = {key: [] for key in ("p", "u", "v", "w", "su", "dens", "mu")}
+    for blk in g.blocks:
+        nx, ny, nz = blk.n
+        gsz = (nx + 2) * (ny + 2) * (nz + 2)
+        for key in ("p", "u", "v", "w"):
+            got[key].append(raw[o:o + gsz].reshape(nz + 2, ny + 2, nx + 2))
+            o += gsz
+        isz = nx * ny * nz
+        got["su"].append(raw[o:o + isz].reshape(nz, ny, nx))
+        o += isz
+        for key in ("dens", "mu"):
+            got[key].append(raw[o:o + gsz].reshape(nz + 2, ny + 2, nx + 2))
+            o += gsz
+        assert raw[o] == 0.0, "the library wrote outside the one-cell ghost layer"
+        o += 1
+    nd = _check_dens(got["dens"], a["dens"], densg, densl)
+    _check_dens(got["mu"], a["mu"], mug, mul)
+    if nd == 0:
+        assert np.array_equal(hist, a["rmax"])
+        for key in ("p", "u", "v", "w", "su"):
+            for x, e in zip(got[key], a[key]):
+                assert np.array_equal(x, e), key
+    else:
+        assert np.allclose(hist, a["rmax"], rtol=1e-9, atol=0)
+        for key in ("p", "u", "v", "w", "su"):
+            assert _rel(got[key], a[key]) < 1e-10, key
