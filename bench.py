@@ -150,6 +150,54 @@ def run_velocity(grid, levels, precision, steps, warmup, rel_eps, maxcy, time_ke
                 eps=eps, status=res.status, u=u, launches=launches, kernel_ms=kms, graph_replay=replay)
 
 
+def levelset_phi(grid):
+    """Signed distance to a sphere of radius 0.3 at the domain centre, over
+    the ghosted box of each block (the transition band |phi| <= 2 dx is
+    about 1.7 % of the cells at 256^3)."""
+    out = []
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        ox, oy, oz = blk.origin
+        ax = [(o + np.arange(m + 2) - 0.5) * h - 0.5 for o, m, h in zip((ox, oy, oz), (nx, ny, nz), grid.h)]
+        r = np.sqrt(ax[2][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[0][None, None, :] ** 2)
+        out.append(0.3 - r)
+    return out
+
+
+def run_levelset(grid, levels, precision, steps, warmup):
+    """The LSM density update of every time step on a resident phi:
+    heaviside + bound_lsm(18/19) + coeff (mf3d_mg_heaviside_device, dens and
+    mu written to caller buffers too).  Steps are timed one by one on the
+    host around the synchronous call; k_heaviside by the kernel hook."""
+    import torch
+
+    from multiflow3d_amd import PressureMultigrid
+    from multiflow3d_amd import _lib
+
+    dev = torch.device("cuda:0")
+    phi = torch.from_numpy(np.concatenate([a.ravel() for a in levelset_phi(grid)])).to(dev)
+    dens = torch.empty_like(phi)
+    mu = torch.empty_like(phi)
+    torch.cuda.synchronize()
+    mg = PressureMultigrid(grid, levels, maxcy=4, precision=precision, variable_coef=True)
+    props = (1000.0, 1.2, 1.0e-3, 1.8e-5)
+    for _ in range(warmup):
+        mg.heaviside_device(phi.data_ptr(), *props, dens_ptr=dens.data_ptr(), mu_ptr=mu.data_ptr())
+    lat = []
+    for _ in range(steps):
+        ts = time.perf_counter()
+        mg.heaviside_device(phi.data_ptr(), *props, dens_ptr=dens.data_ptr(), mu_ptr=mu.data_ptr())
+        lat.append((time.perf_counter() - ts) * 1e3)
+    mg.time_kernel(_lib.MF3D_K_HEAVISIDE)
+    for _ in range(steps):
+        mg.heaviside_device(phi.data_ptr(), *props, dens_ptr=dens.data_ptr(), mu_ptr=mu.data_ptr())
+    launches, kms = mg.kernel_time()
+    mg.time_kernel(0)
+    mg.close()
+    band = int((phi.abs() <= 2 * max(grid.h)).sum().item())
+    return dict(ms=float(np.mean(lat)), lat=lat, launches=launches, kernel_ms=kms, cells=phi.numel(), band=band)
+
+
 def run_serving(count):
     """Serving sweep: Poisson arrivals of independent solves, n^3 with n
     uniform in {32,64,96,128,192,256}, tolerance uniform in {1e-6,1e-8,1e-10},
@@ -194,6 +242,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-velocity", action="store_true",
                     help="skip the velocity-form (calvel/calmas) leg")
+    ap.add_argument("--no-levelset", action="store_true",
+                    help="skip the level-set density update (heaviside + coeff) leg")
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests per arrival rate in the serving sweep (0: skip it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -328,6 +378,29 @@ def main():
                            f"rmax < {args.tol:g} x rmax_0")
         velocity = vel
 
+    levelset = None
+    if not args.no_levelset:
+        levelset = {}
+        for prec in ("fp64", "mixed"):
+            r = run_levelset(grid, L, prec, args.steps, args.warmup)
+            cg = r["cells"]
+            per_launch = 32 * cg  # reads phi, writes the level-0 density, dens and mu (8 B each)
+            avg_s = r["kernel_ms"] / r["launches"] * 1e-3 if r["launches"] else None
+            levelset[prec] = {
+                "ms_per_update": round(r["ms"], 4),
+                "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+                "k_heaviside": None if avg_s is None else {
+                    "bound": "hbm", "avg_us": round(avg_s * 1e6, 2), "launches_timed": r["launches"],
+                    "bytes_per_launch": int(per_launch),
+                    "achieved": round(per_launch / avg_s / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(per_launch / avg_s / 1e9 / HBM_PEAK_GBPS, 4),
+                    "model": "reads phi, writes the handle's level-0 density and the caller's dens and "
+                             "mu, 8 B each per ghosted cell"}}
+            print(f"# levelset {prec}: {r['ms']:.3f} ms/update (heaviside + coeff), band {r['band']} of "
+                  f"{cg} cells", file=sys.stderr, flush=True)
+        levelset["workload"] = (f"heaviside + bound_lsm(18/19) + coeff per time step on the same {n}^3 "
+                                f"grid, {L} levels: phi = signed distance to a sphere r = 0.3, "
+                                f"densl/densg 1000/1.2, mul/mug 1e-3/1.8e-5, dens and mu returned")
     serving = run_serving(args.serving_requests) if args.serving_requests > 0 else None
 
     lat = np.array(rmx["lat"])
@@ -364,6 +437,7 @@ def main():
                   "rel_l2_vs_fp64": mixed_err, "graph_replay": rmx["graph_replay"]},
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
         "velocity_form": velocity,
+        "levelset_update": levelset,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -48,8 +48,8 @@ constexpr int kTileX = 64, kTileY = 4;
 struct Level {
     std::vector<BlockLevel> hb;
     BlockLevel *db = nullptr;
-    Tile *t_sw = nullptr, *t_in = nullptr, *t_row = nullptr, *t_oct = nullptr;
-    int n_sw = 0, n_in = 0, n_row = 0, n_oct = 0;
+    Tile *t_sw = nullptr, *t_in = nullptr, *t_row = nullptr, *t_oct = nullptr, *t_ch = nullptr;
+    int n_sw = 0, n_in = 0, n_row = 0, n_oct = 0, n_ch = 0;
     int sw_ty = 4;  // sweep tile rows
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
@@ -295,6 +295,18 @@ std::vector<Tile> row_tiles(const std::vector<BlockLevel> &hb) {
         const BlockLevel &g = hb[b];
         for (int z = 0; z <= g.n2 + 1; ++z)
             for (int y = 0; y <= g.n1 + 1; y += kTileY) t.push_back(Tile{b, 0, y, z, z});
+    }
+    return t;
+}
+
+// Chunks of kCoefChunk consecutive entries of each block's ghosted arena
+// (x0 = the chunk's first entry within the block).
+std::vector<Tile> chunk_tiles(const std::vector<BlockLevel> &hb) {
+    std::vector<Tile> t;
+    for (int b = 0; b < (int)hb.size(); ++b) {
+        const BlockLevel &g = hb[b];
+        const long long cells = g.sz * (g.n2 + 2);
+        for (long long c = 0; c < cells; c += kCoefChunk) t.push_back(Tile{b, (int)c, 0, 0, 0});
     }
     return t;
 }
@@ -898,6 +910,9 @@ int build(mf3d_mg *mg) {
         auto tsw = xcd_order(sweep_tiles(V.hb, V.sw_ty)), tin = xcd_order(cell_tiles(V.hb)),
              trow = xcd_order(row_tiles(V.hb));
         auto toct = xcd_order(octet_tiles(V.hb));
+        auto tch = xcd_order(chunk_tiles(V.hb));
+        V.n_ch = (int)tch.size();
+        if ((rc = upload(mg, &V.t_ch, tch))) return rc;
         V.n_sw = (int)tsw.size(); V.n_in = (int)tin.size(); V.n_row = (int)trow.size();
         V.n_oct = (int)toct.size();
         if ((rc = upload(mg, &V.t_oct, toct))) return rc;
@@ -1100,9 +1115,13 @@ int build_lsm(mf3d_mg *mg) {
             const Level &F = mg->lv[l - 1];
             hipLaunchKernelGGL(k_face_restrict, dim3(V.n_row), dim3(kTileX, kTileY), 0, mg->stream, F.dcx, F.dcy,
                                F.dcz, F.db, V.dcx, V.dcy, V.dcz, V.db, V.t_row);
+        } else if (mg->cfg.precision == MF3D_MIXED) {
+            hipLaunchKernelGGL(k_lsm_coef<true>, dim3(V.n_ch), dim3(256), 0, mg->stream, V.dens, V.dcx, V.dcy,
+                               V.dcz, (float *)V.cx, (float *)V.cy, (float *)V.cz, V.db, V.t_ch);
+            continue;  // single-precision copies written by the same kernel
         } else {
-            hipLaunchKernelGGL(k_lsm_coef, dim3(V.n_row), dim3(kTileX, kTileY), 0, mg->stream, V.dens, V.dcx,
-                               V.dcy, V.dcz, V.db, V.t_row);
+            hipLaunchKernelGGL(k_lsm_coef<false>, dim3(V.n_ch), dim3(256), 0, mg->stream, V.dens, V.dcx, V.dcy,
+                               V.dcz, (float *)nullptr, (float *)nullptr, (float *)nullptr, V.db, V.t_ch);
         }
         if (mg->cfg.precision == MF3D_MIXED) {
             convert<float, double>(mg, (float *)V.cx, V.dcx, V.cells_gh);

@@ -71,10 +71,14 @@ __global__ __launch_bounds__(256) void k_heaviside(const double *__restrict__ ph
         const long long c = base + u * 256;
         v[u] = c < B.cells ? phi[B.off + c] : 0.0;
     }
+    // every h before the first store (a load consumed after a store makes
+    // the wait for it wait for the store too)
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) v[u] = ls_h(v[u], B.epsl);
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const long long c = base + u * 256;
-        if (c < B.cells) ls_store(B.off + c, ls_h(v[u], B.epsl), pr, dens, dens_out, mu);
+        if (c < B.cells) ls_store(B.off + c, v[u], pr, dens, dens_out, mu);
     }
 }
 

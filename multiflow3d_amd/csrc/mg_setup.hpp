@@ -89,51 +89,91 @@ __global__ __launch_bounds__(256) void k_face_restrict(const double *__restrict_
 // the mean of the two cells (the cell itself at the block edge); cx at
 // i = n0+1 is ae of the last cell (one-sided).  A physical Neumann face
 // gets 0.  The finest dx, dy, dz are used on every level, as the
-// reference does.  Row tiles over the ghosted range.
-__global__ __launch_bounds__(256) void k_lsm_coef(const double *__restrict__ d,
-                                                  double *__restrict__ cx, double *__restrict__ cy,
-                                                  double *__restrict__ cz,
-                                                  const BlockLevel *__restrict__ bls,
-                                                  const Tile *__restrict__ tiles) {
-    const Tile t = tiles[blockIdx.x];
+// reference does.  Entries outside these ranges are not written (they stay
+// zero).
+//
+// A workgroup streams kCoefChunk consecutive entries of one block's
+// ghosted arena (whole rows and planes, i fastest), each thread four
+// entries 256 apart with all loads issued first; the chunks are ordered so
+// that each XCD walks a contiguous range of planes, and the plane below
+// (read for cz) is still in that XCD's L2.  With F32 the single-precision
+// copies of the inner cycle are written in the same pass, every entry (0
+// where the FP64 coefficient is not written), as a separate conversion
+// of the whole arrays would.
+constexpr int kCoefChunk = 1024;
+
+template <bool F32>
+__global__ __launch_bounds__(256) void k_lsm_coef(const double *__restrict__ d, double *__restrict__ cx,
+                                                  double *__restrict__ cy, double *__restrict__ cz,
+                                                  float *__restrict__ fx, float *__restrict__ fy,
+                                                  float *__restrict__ fz, const BlockLevel *__restrict__ bls,
+                                                  const Tile *__restrict__ chunks) {
+    constexpr int U = kCoefChunk / 256;
+    const Tile t = chunks[blockIdx.x];
     const BlockLevel g = bls[t.b];
-    const int j = t.y0 + threadIdx.y, k = t.z0;
-    if (j > g.n1 + 1) return;
-    for (int i = threadIdx.x; i <= g.n0 + 1; i += blockDim.x) {
-        const long long q = gidx(g, i, j, k);
-        const bool jin = j >= 1 && j <= g.n1, kin = k >= 1 && k <= g.n2, iin = i >= 1 && i <= g.n0;
-        if (jin && kin && i >= 1) {
-            double v;
-            if (i <= g.n0) {
-                const double d0 = d[q];
-                const double dm = (i == 1) ? d0 : 0.5 * (d0 + d[q - 1]);
-                v = (i == 1 && (g.neu & kNeuW)) ? 0.0 : g.gx / dm;
-            } else {
-                v = (g.neu & kNeuE) ? 0.0 : g.gx / d[q - 1];
+    const int sy = (int)g.sy, ny2 = g.n1 + 2;
+    const long long sz = g.sz;
+    const int cells = (int)(g.sz * (g.n2 + 2));
+    const int c0 = t.x0 + (int)threadIdx.x;
+    int i = c0 % sy, r = c0 / sy;
+    int j = r % ny2, k = r / ny2;
+    int ci[U], cj[U], ck[U];
+    double v0[U], vx[U], vy[U], vz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        ci[u] = i;
+        cj[u] = j;
+        ck[u] = k;
+        const int c = c0 + 256 * u;
+        const bool ok = c < cells;
+        const long long q = g.off + c;
+        v0[u] = ok ? d[q] : 1.0;
+        vx[u] = ok && i >= 1 ? d[q - 1] : 1.0;
+        vy[u] = ok && j >= 1 ? d[q - sy] : 1.0;
+        vz[u] = ok && k >= 1 ? d[q - sz] : 1.0;
+        i += 256;
+        while (i >= sy) {
+            i -= sy;
+            if (++j == ny2) {
+                j = 0;
+                ++k;
             }
-            cx[q] = v;
         }
-        if (iin && kin && j >= 1) {
-            double v;
-            if (j <= g.n1) {
-                const double d0 = d[q];
-                const double dm = (j == 1) ? d0 : 0.5 * (d0 + d[q - g.sy]);
-                v = (j == 1 && (g.neu & kNeuS)) ? 0.0 : g.gy / dm;
-            } else {
-                v = (g.neu & kNeuN) ? 0.0 : g.gy / d[q - g.sy];
-            }
-            cy[q] = v;
-        }
-        if (iin && jin && k >= 1) {
-            double v;
-            if (k <= g.n2) {
-                const double d0 = d[q];
-                const double dm = (k == 1) ? d0 : 0.5 * (d0 + d[q - g.sz]);
-                v = (k == 1 && (g.neu & kNeuB)) ? 0.0 : g.gz / dm;
-            } else {
-                v = (g.neu & kNeuT) ? 0.0 : g.gz / d[q - g.sz];
-            }
-            cz[q] = v;
+    }
+    // all arithmetic before the first store: a load consumed after a store
+    // would make the wait for it wait for that store as well
+    double ox[U], oy[U], oz[U];
+    bool wx[U], wy[U], wz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int ii = ci[u], jj = cj[u], kk = ck[u];
+        const bool ok = c0 + 256 * u < cells;
+        const bool jin = jj >= 1 && jj <= g.n1, kin = kk >= 1 && kk <= g.n2, iin = ii >= 1 && ii <= g.n0;
+        wx[u] = ok && jin && kin && ii >= 1;
+        wy[u] = ok && iin && kin && jj >= 1;
+        wz[u] = ok && iin && jin && kk >= 1;
+        // i <= n0: the face mean (the cell itself at i = 1); i = n0+1: the
+        // last cell (one-sided); 0 on a physical Neumann face
+        const double mx = ii <= g.n0 ? (ii == 1 ? v0[u] : 0.5 * (v0[u] + vx[u])) : vx[u];
+        const double my = jj <= g.n1 ? (jj == 1 ? v0[u] : 0.5 * (v0[u] + vy[u])) : vy[u];
+        const double mz = kk <= g.n2 ? (kk == 1 ? v0[u] : 0.5 * (v0[u] + vz[u])) : vz[u];
+        const bool zx = ii <= g.n0 ? (ii == 1 && (g.neu & kNeuW)) : (g.neu & kNeuE) != 0;
+        const bool zy = jj <= g.n1 ? (jj == 1 && (g.neu & kNeuS)) : (g.neu & kNeuN) != 0;
+        const bool zz = kk <= g.n2 ? (kk == 1 && (g.neu & kNeuB)) : (g.neu & kNeuT) != 0;
+        ox[u] = zx ? 0.0 : g.gx / mx;
+        oy[u] = zy ? 0.0 : g.gy / my;
+        oz[u] = zz ? 0.0 : g.gz / mz;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const long long q = g.off + c0 + 256 * u;
+        if (wx[u]) cx[q] = ox[u];
+        if (wy[u]) cy[q] = oy[u];
+        if (wz[u]) cz[q] = oz[u];
+        if (F32 && c0 + 256 * u < cells) {
+            fx[q] = wx[u] ? (float)ox[u] : 0.0f;
+            fy[q] = wy[u] ? (float)oy[u] : 0.0f;
+            fz[q] = wz[u] ? (float)oz[u] : 0.0f;
         }
     }
 }
