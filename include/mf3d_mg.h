@@ -89,7 +89,8 @@ typedef struct {
 typedef struct {
     int32_t nblocks;       /* nbp */
     int32_t levels;        /* ngrd_gl; every extent must divide by 2^(levels-1) */
-    int32_t smoother;      /* mg_itrsch; only 3 (red-black Gauss-Seidel) is supported */
+    int32_t smoother;      /* mg_itrsch (mgsolver.for:403-611): 1 lexicographic Gauss-Seidel,
+                              2 ADI line relaxation with TDMA, 3 red-black Gauss-Seidel */
     int32_t nu_pre;        /* irestr */
     int32_t nu_post;       /* iproln */
     int32_t maxcy;         /* maximum outer cycles */

@@ -27,6 +27,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mf3d_mg.h"
@@ -37,6 +38,7 @@
 #include "mg_tail.hpp"
 #include "mg_transfer.hpp"
 #include "mg_levelset.hpp"
+#include "mg_linesmooth.hpp"
 #include "mg_velocity.hpp"
 
 using namespace mf3d;
@@ -146,6 +148,9 @@ struct mf3d_mg {
     LsBlock *d_lsb = nullptr;
     long long ls_max_cells = 0;
     double *d_lsio[2] = {nullptr, nullptr};
+    // ADI smoother (mg_itrsch = 2): the Thomas a(m) of the lines in flight,
+    // at the offsets of their cells (level-0 size, shared by all levels)
+    void *d_adi = nullptr;
 };
 
 namespace {
@@ -193,7 +198,8 @@ bool is_neumann(const mf3d_block_desc &b, int f) { return b.nbr[f] < 0 && b.bc[f
 std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (c.nblocks <= 0) return "nblocks must be positive";
     if (c.levels <= 0 || c.levels > 16) return "levels out of range";
-    if (c.smoother != 3) return "only the red-black smoother (mg_itrsch = 3) is supported";
+    if (c.smoother < 1 || c.smoother > 3)
+        return "unknown smoother (mg_itrsch: 1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS)";
     if (c.nu_pre < 0 || c.nu_post < 0 || c.maxcy < 0) return "negative sweep or cycle count";
     if (c.precision != MF3D_FP64 && c.precision != MF3D_MIXED) return "unknown precision";
     if (c.coarse_op != MF3D_COARSE_REFERENCE && c.coarse_op != MF3D_COARSE_FACE) return "unknown coarse_op";
@@ -384,6 +390,7 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     }
     tot += gh0 * 8 * 3;             // p (two buffers) and f
     if (c.krylov == MF3D_OUTER_GCR) tot += gh0 * 8 * 4;  // d (two buffers), w, r
+    if (c.smoother == 2) tot += gh0 * w;                 // ADI's a(m) scratch
     tot += std::max(gh0, in0) * 8;  // I/O staging
     return tot;
 }
@@ -473,11 +480,64 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
                            L.t_sw, done);
 }
 
+// One in-place sweep of the lexicographic (1) or ADI (2) smoother on level
+// l's current potential: a launch per hyperplane, or per line diagonal and
+// direction (j, i, k lines, mgsolver.for:433-535).
+template <typename T, bool VAR>
+void sweep_inplace(mf3d_mg *mg, Level &L, const int *done) {
+    int m0 = 1, m1 = 1, m2 = 1, nb = mg->cfg.nblocks;
+    for (const BlockLevel &g : L.hb) {
+        m0 = std::max(m0, g.n0);
+        m1 = std::max(m1, g.n1);
+        m2 = std::max(m2, g.n2);
+    }
+    T *p = (T *)L.phi[L.cur];
+    const T *rhs = (const T *)L.rhs, *cx = (const T *)L.cx, *cy = (const T *)L.cy, *cz = (const T *)L.cz;
+    if (mg->cfg.smoother == 1) {
+        constexpr bool RCP = sizeof(T) == 4;
+        const dim3 grid((unsigned)((m1 + 15) / 16), (unsigned)((m2 + 15) / 16), (unsigned)nb);
+        int smax = 0;
+        for (const BlockLevel &g : L.hb) smax = std::max(smax, g.n0 + g.n1 + g.n2);
+        for (int s = 3; s <= smax; ++s)
+            hipLaunchKernelGGL((k_lex<T, VAR, RCP>), grid, dim3(256), 0, mg->stream, p, rhs, cx, cy, cz, L.db, s,
+                               done);
+        return;
+    }
+    T *sa = (T *)mg->d_adi;
+    auto pass = [&](auto ax, int nu_max, int nv_max, int lines_max) {
+        constexpr int AX = decltype(ax)::value;
+        const dim3 grid((unsigned)((lines_max + 63) / 64), (unsigned)nb);
+        for (int d = 2; d <= nu_max + nv_max; ++d)
+            hipLaunchKernelGGL((k_adi<T, VAR, AX>), grid, dim3(64), 0, mg->stream, p, sa, rhs, cx, cy, cz, L.db, d,
+                               done);
+    };
+    int lj = 1, li = 1, lk = 1;  // the most lines one diagonal of a pass holds
+    for (const BlockLevel &g : L.hb) {
+        lj = std::max(lj, std::min(g.n0, g.n2));
+        li = std::max(li, std::min(g.n1, g.n2));
+        lk = std::max(lk, std::min(g.n0, g.n1));
+    }
+    pass(std::integral_constant<int, 1>{}, m0, m2, lj);  // j lines, (u, v) = (i, k)
+    pass(std::integral_constant<int, 0>{}, m1, m2, li);  // i lines, (j, k)
+    pass(std::integral_constant<int, 2>{}, m1, m0, lk);  // k lines, (j, i)
+}
+
 // `sweeps` sweeps at level l; the first treats the potential as zero when
 // zero_first (after restriction, or at the start of a cycle from pp = 0).
 template <typename T, bool VAR>
 void relax(mf3d_mg *mg, int l, int sweeps, bool zero_first, const int *done) {
     Level &L = mg->lv[l];
+    if (mg->cfg.smoother != 3) {  // in place, from an explicit zero
+        if (zero_first) fill<T>(mg, (T *)L.phi[L.cur], L.cells_gh, done);
+        for (int s = 0; s < sweeps; ++s) {
+            const bool tk = tk_on(mg, MF3D_K_SMOOTH, l);
+            if (tk) tk_mark(mg);
+            sweep_inplace<T, VAR>(mg, L, done);
+            if (tk) tk_mark(mg);
+            gather<T>(mg, l, done);
+        }
+        return;
+    }
     if (sweeps == 0 && zero_first) {  // nothing will overwrite the zero the reference leaves
         fill<T>(mg, (T *)L.phi[L.cur], L.cells_gh, done);
         return;
@@ -817,6 +877,7 @@ int choose_tail(mf3d_mg *mg) {
     mg->lt = L;
     const char *env = std::getenv("MF3D_TAIL");
     if (env && env[0] == '0') return 0;
+    if (mg->cfg.smoother != 3) return 0;  // the tail smooths red-black
     for (const mf3d_block_desc &B : mg->blk)
         for (int f = 0; f < 6; ++f)
             if (B.nbr[f] >= 0 || B.per[f] >= 0) return 0;
@@ -930,7 +991,7 @@ int build(mf3d_mg *mg) {
                     lone = lone && dim_at(B, d, l) >= 2 && is_neumann(B, 2 * d) == is_neumann(B, 2 * d + 1);
             }
             const char *env = std::getenv("MF3D_OWN_GHOSTS");
-            V.own_ghosts = lone && !(env && env[0] == '0');
+            V.own_ghosts = lone && c.smoother == 3 && !(env && env[0] == '0');
         }
         if ((rc = upload(mg, &V.r_full, rf))) return rc;
         if ((rc = dev_alloc(mg, &V.phi[0], V.cells_gh * w)) || (rc = dev_alloc(mg, &V.phi[1], V.cells_gh * w)) ||
@@ -997,6 +1058,8 @@ int build(mf3d_mg *mg) {
     hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st, mg->cfg.eps);
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
+    if (c.smoother == 2 && (rc = dev_alloc(mg, &mg->d_adi, L0.cells_gh * w))) return rc;
+    if (c.smoother != 3) return 0;  // the tiles tuned below belong to the red-black sweep
     if ((rc = mixed ? (c.variable_coef ? autotune_sweeps<float, true>(mg) : autotune_sweeps<float, false>(mg))
                     : (c.variable_coef ? autotune_sweeps<double, true>(mg) : autotune_sweeps<double, false>(mg))))
         return rc;
@@ -1758,7 +1821,6 @@ int mf3d_mg_create(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, mf3d_m
     if (!out) return MF3D_EINVAL;
     *out = nullptr;
     if (!cfg || !blocks) return MF3D_EINVAL;
-    if (cfg->smoother != 3) return MF3D_ENOTSUP;
     std::string v = validate(*cfg, blocks);
     mf3d_mg *mg = new mf3d_mg();
     mg->cfg = *cfg;

@@ -8,7 +8,9 @@
  * i fastest), the arithmetic and the order of operations of:
  *   coeff    mgsolver.for:2-222     coefficient hierarchy (constant and LSM)
  *   mgkcyc   mgsolver.for:224-369   V-cycle driven by the K-cycle counter
- *   mgrelax  mgsolver.for:547-611   red-black Gauss-Seidel (mg_itrsch = 3)
+ *   mgrelax  mgsolver.for:371-617   lexicographic Gauss-Seidel (mg_itrsch = 1),
+ *                                   ADI line relaxation with TDMA (2),
+ *                                   red-black Gauss-Seidel (3)
  *   mgrestr  mgsolver.for:619-761   residual + 8-cell average restriction
  *   mgcorr   mgsolver.for:763-905   trilinear prolongation + correction
  *   mgbound  mgsolver.for:907-992   Neumann ghost copy
@@ -20,7 +22,6 @@
  * reference.  Build with -ffp-contract=off.
  *
  * Scope notes (what is not restated):
- *  - smoothers 1 and 2 (lexicographic GS, ADI-TDMA): mgsolver.for:403-545;
  *  - refined-block (rdv unequal) exchange branches and the injection branch
  *    for blocks with fewer levels (mgsolver.for:719-732, 807-819);
  *  - the explicit corner/edge messages of exchangepp.for:433-1093: on a
@@ -39,6 +40,8 @@ enum { FW = 0, FE = 1, FS = 2, FN = 3, FB = 4, FT = 5 };
 
 struct orc_mg {
     int nb, L, lsm, nu_pre, nu_post, periodic;
+    int itrsch;         /* mg_itrsch: 1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS */
+    double *ta, *tc;    /* TDMA recurrence arrays a(), c() of mgsolver.for:381 */
     orc_block *blk;
     int32_t (*dims)[3]; /* [b*L + l] interior dims */
     double **phi;       /* [b*L + l] ghosted */
@@ -94,7 +97,8 @@ orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
     mg->bufp = calloc(nblocks, sizeof(double *));
     mg->pf = calloc(nblocks, sizeof(double *));
     mg->periodic = 0;
-    size_t face_max = 0;
+    mg->itrsch = 3;
+    size_t face_max = 0, line_max = 0;
     for (int b = 0; b < nblocks; ++b) {
         for (int f = 0; f < 6; ++f)
             if (blocks[b].bc[f] == 5) mg->periodic = 1; /* alloc_dom.for:1156-1163 */
@@ -113,7 +117,11 @@ orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
         fm = fm > f12 ? fm : f12;
         if (fm > face_max) face_max = fm;
         mg->pf[b] = calloc(gsz(n0), sizeof(double));
+        for (int d = 0; d < 3; ++d)
+            if ((size_t)n0[d] + 2 > line_max) line_max = (size_t)n0[d] + 2;
     }
+    mg->ta = calloc(line_max, sizeof(double));
+    mg->tc = calloc(line_max, sizeof(double));
     for (int b = 0; b < nblocks; ++b) {
         mg->bufm[b] = calloc(face_max, sizeof(double));
         mg->bufp[b] = calloc(face_max, sizeof(double));
@@ -123,6 +131,8 @@ orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
 
 void orc_destroy(orc_mg *mg) {
     if (!mg) return;
+    free(mg->ta);
+    free(mg->tc);
     size_t ns = (size_t)mg->nb * mg->L;
     for (size_t s = 0; s < ns; ++s) {
         free(mg->phi[s]);
@@ -419,13 +429,130 @@ static void rb_colour(orc_mg *mg, int b, int l, int parity) {
             }
 }
 
+/* mgrelax, mg_itrsch = 1 (mgsolver.for:403-428): Gauss-Seidel in the loop
+ * order k, j, i (i innermost), updating in place, so the W, S and B
+ * neighbours are this sweep's values and E, N, T the previous ones. */
+static void lex_block(orc_mg *mg, int b, int l) {
+    const int32_t *n = DIMS(mg, b, l);
+    size_t N = isz(n);
+    double *p = mg->phi[SLOT(mg, b, l)];
+    const double *c = mg->cof[SLOT(mg, b, l)];
+    const double *rh = mg->rhs[SLOT(mg, b, l)];
+    size_t sy = (size_t)(n[0] + 2), sz = sy * (size_t)(n[1] + 2);
+    for (int k = 1; k <= n[2]; ++k)
+        for (int j = 1; j <= n[1]; ++j)
+            for (int i = 1; i <= n[0]; ++i) {
+                size_t q = ci(n, i, j, k), g = gi(n, i, j, k);
+                double s = c[q] * p[g - 1];
+                s = s + c[N + q] * p[g + 1];
+                s = s + c[2 * N + q] * p[g - sy];
+                s = s + c[3 * N + q] * p[g + sy];
+                s = s + c[4 * N + q] * p[g - sz];
+                s = s + c[5 * N + q] * p[g + sz];
+                p[g] = (rh[q] - s) / c[6 * N + q];
+            }
+}
+
+/* mgrelax, mg_itrsch = 2 (mgsolver.for:430-545): TDMA along j lines (lines
+ * in the order k, i), then along i lines (k, j), then along k lines (i, j),
+ * each line solved in place with the Thomas algorithm; the other
+ * directions' neighbours are read as they stand, so lines earlier in the
+ * pass contribute their new values.  a(1) = 0 and c(1) = the ghost before
+ * the line; apr = 1/(ap - a_lo*a(m-1)), a(m) = a_hi*apr,
+ * c(m) = (su - the four cross terms in the reference's order - a_lo*c(m-1))*apr,
+ * then phi(m) = c(m) - a(m)*phi(m+1) from the ghost after the line down. */
+static void adi_block(orc_mg *mg, int b, int l) {
+    const int32_t *n = DIMS(mg, b, l);
+    size_t N = isz(n);
+    double *p = mg->phi[SLOT(mg, b, l)];
+    const double *c = mg->cof[SLOT(mg, b, l)];
+    const double *rh = mg->rhs[SLOT(mg, b, l)];
+    double *a = mg->ta, *cc = mg->tc;
+    const double *aw = c, *ae = c + N, *as = c + 2 * N, *an = c + 3 * N, *ab = c + 4 * N, *at = c + 5 * N,
+                 *ap = c + 6 * N;
+    size_t sy = (size_t)(n[0] + 2), sz = sy * (size_t)(n[1] + 2);
+    a[0] = 0.0;
+    /* j lines (:433-463) */
+    for (int k = 1; k <= n[2]; ++k)
+        for (int i = 1; i <= n[0]; ++i) {
+            cc[0] = p[gi(n, i, 0, k)];
+            for (int j = 1; j <= n[1]; ++j) {
+                size_t q = ci(n, i, j, k), g = gi(n, i, j, k);
+                double apr = 1.0 / (ap[q] - as[q] * a[j - 1]);
+                a[j] = an[q] * apr;
+                double t = rh[q] - ae[q] * p[g + 1];
+                t = t - aw[q] * p[g - 1];
+                t = t - at[q] * p[g + sz];
+                t = t - ab[q] * p[g - sz];
+                t = t - as[q] * cc[j - 1];
+                cc[j] = t * apr;
+            }
+            for (int j = n[1]; j >= 1; --j) {
+                size_t g = gi(n, i, j, k);
+                p[g] = cc[j] - a[j] * p[g + sy];
+            }
+        }
+    /* i lines (:466-499) */
+    for (int k = 1; k <= n[2]; ++k)
+        for (int j = 1; j <= n[1]; ++j) {
+            cc[0] = p[gi(n, 0, j, k)];
+            for (int i = 1; i <= n[0]; ++i) {
+                size_t q = ci(n, i, j, k), g = gi(n, i, j, k);
+                double apr = 1.0 / (ap[q] - aw[q] * a[i - 1]);
+                a[i] = ae[q] * apr;
+                double t = rh[q] - an[q] * p[g + sy];
+                t = t - as[q] * p[g - sy];
+                t = t - at[q] * p[g + sz];
+                t = t - ab[q] * p[g - sz];
+                t = t - aw[q] * cc[i - 1];
+                cc[i] = t * apr;
+            }
+            for (int i = n[0]; i >= 1; --i) {
+                size_t g = gi(n, i, j, k);
+                p[g] = cc[i] - a[i] * p[g + 1];
+            }
+        }
+    /* k lines (:502-535) */
+    for (int i = 1; i <= n[0]; ++i)
+        for (int j = 1; j <= n[1]; ++j) {
+            cc[0] = p[gi(n, i, j, 0)];
+            for (int k = 1; k <= n[2]; ++k) {
+                size_t q = ci(n, i, j, k), g = gi(n, i, j, k);
+                double apr = 1.0 / (ap[q] - ab[q] * a[k - 1]);
+                a[k] = at[q] * apr;
+                double t = rh[q] - an[q] * p[g + sy];
+                t = t - as[q] * p[g - sy];
+                t = t - ae[q] * p[g + 1];
+                t = t - aw[q] * p[g - 1];
+                t = t - ab[q] * cc[k - 1];
+                cc[k] = t * apr;
+            }
+            for (int k = n[2]; k >= 1; --k) {
+                size_t g = gi(n, i, j, k);
+                p[g] = cc[k] - a[k] * p[g + sz];
+            }
+        }
+}
+
+int orc_set_smoother(orc_mg *mg, int itrsch) {
+    if (itrsch < 1 || itrsch > 3) return -1;
+    mg->itrsch = itrsch;
+    return 0;
+}
+
 void orc_relax(orc_mg *mg, int l, int sweeps) {
     double **f = malloc(sizeof(double *) * mg->nb);
     level_phis(mg, l, f);
     for (int it = 0; it < sweeps; ++it) {
         for (int b = 0; b < mg->nb; ++b) {
-            rb_colour(mg, b, l, 1);
-            rb_colour(mg, b, l, 0);
+            if (mg->itrsch == 1) {
+                lex_block(mg, b, l);
+            } else if (mg->itrsch == 2) {
+                adi_block(mg, b, l);
+            } else {
+                rb_colour(mg, b, l, 1);
+                rb_colour(mg, b, l, 0);
+            }
         }
         bound_fields(mg, l, f);   /* mgsolver.for:610 */
         exchange_fields(mg, l, f); /* mgsolver.for:611 */

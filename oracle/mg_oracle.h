@@ -35,6 +35,9 @@ typedef struct orc_mg orc_mg;
 orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
                    int nu_pre, int nu_post);
 void    orc_destroy(orc_mg *mg);
+/* Smoother (mg_itrsch): 1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS
+ * (the default).  Returns 0, or -1 for another value. */
+int     orc_set_smoother(orc_mg *mg, int itrsch);
 /* Level-0 cell density (interior, concatenated).  Only for lsm handles. */
 int     orc_set_density(orc_mg *mg, const double *dens);
 /* Coefficient hierarchy (mgsolver.for:2-222). */

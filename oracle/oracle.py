@@ -41,6 +41,8 @@ def lib():
         L.orc_create.argtypes = [ip, ctypes.POINTER(OrcBlock), ip, ip, ip, ip]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_density.argtypes = [vp, dp]
+        L.orc_set_smoother.restype = ip
+        L.orc_set_smoother.argtypes = [vp, ip]
         L.orc_coeff.argtypes = [vp]
         L.orc_vcycle.argtypes = [vp, dp, dp]
         L.orc_solve.restype = ip
@@ -75,10 +77,11 @@ class OracleMG:
     """FP64 multigrid of the reference, restated on the CPU."""
 
     def __init__(self, blocks, levels: int, lsm: bool = False, nu_pre: int = 2, nu_post: int = 2,
-                 coarse_op: str = "reference"):
+                 coarse_op: str = "reference", smoother: int = 3):
         """lsm: variable (density) coefficients; coarse_op "face" builds their
         coarse operators by face-coefficient restriction instead of the
-        reference's density hierarchy."""
+        reference's density hierarchy; smoother: mg_itrsch (1 lexicographic
+        GS, 2 ADI-TDMA, 3 red-black GS)."""
         self.blocks = list(blocks)
         self.levels = levels
         self.lsm = lsm
@@ -93,6 +96,8 @@ class OracleMG:
         self._h = lib().orc_create(len(self.blocks), arr, levels, mode, nu_pre, nu_post)
         if not self._h:
             raise ValueError("oracle rejected the block set (extent not divisible by 2**(L-1)?)")
+        if lib().orc_set_smoother(self._h, smoother) != 0:
+            raise ValueError(f"unknown smoother {smoother}")
 
     def close(self):
         if self._h:
@@ -203,8 +208,9 @@ def have_ref() -> bool:
 
 
 def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int = 1,
-            lsm: bool = False, dens_blocks=None, nu_pre: int = 2, nu_post: int = 2):
-    """Run the compiled reference driver.  Returns (p_blocks, rho, rmax, cycles, pp1_blocks)."""
+            lsm: bool = False, dens_blocks=None, nu_pre: int = 2, nu_post: int = 2, smoother: int = 3):
+    """Run the compiled reference driver (mg_itrsch = smoother).  Returns
+    (p_blocks, rho, rmax, cycles, pp1_blocks)."""
     if not have_ref():
         raise RuntimeError("compiled reference not built (make -C oracle ref)")
     with tempfile.TemporaryDirectory() as td:
@@ -219,7 +225,7 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
         with open(ctl, "w") as fh:
             nbx, nby, nbz = grid.shape
             fh.write(f"{nbx} {nby} {nbz} {levels} {int(lsm)} {maxcy} {min_cycles} {nu_pre} {nu_post}\n")
-            fh.write(f"{tol!r}\n")
+            fh.write(f"{tol!r} {smoother}\n")
             fh.write(" ".join(str(c) for c in grid.bc) + "\n")
             fh.write(" ".join(map(str, grid.ex)) + "\n")
             fh.write(" ".join(map(str, grid.ey)) + "\n")
@@ -248,7 +254,8 @@ def have_ref_vel() -> bool:
     return os.path.exists(REF_VEL_BIN)
 
 
-def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None, nu_pre=2, nu_post=2):
+def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None, nu_pre=2, nu_post=2,
+                  smoother=3):
     """One run of mg_ref_vel in `mode` (0 constant, 1 density, 2 level set,
     3 bound_lsm only) on the concatenated input `parts`; returns its raw
     f64 output."""
@@ -262,7 +269,7 @@ def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None,
         with open(ctl, "w") as fh:
             nbx, nby, nbz = grid.shape
             fh.write(f"{nbx} {nby} {nbz} {levels} {mode} {maxcy} {nu_pre} {nu_post}\n")
-            fh.write(f"{eps!r}\n")
+            fh.write(f"{eps!r} {smoother}\n")
             fh.write(" ".join(str(c) for c in grid.bc) + "\n")
             fh.write(" ".join(map(str, grid.ex)) + "\n")
             fh.write(" ".join(map(str, grid.ey)) + "\n")
@@ -289,7 +296,7 @@ def _boxes(grid, raw, o, keys):
 
 def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: int, dt: float,
                 alfapr: float, lsm: bool = False, dens=None, nu_pre: int = 2, nu_post: int = 2,
-                phi=None, props=None):
+                phi=None, props=None, smoother: int = 3):
     """Run the compiled reference's velocity-form newsolv_mg loop (calvel,
     calmas, mgkcyc, p update, exchange(4), pbound).  Every field is a list of
     full ghosted block arrays (one ghost layer), shaped (z+2, y+2, x+2).
@@ -305,7 +312,7 @@ def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: i
             parts.append(dens[b])
         elif mode == 2:
             parts.append(phi[b])
-    raw = _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props, nu_pre, nu_post)
+    raw = _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props, nu_pre, nu_post, smoother)
     k = int(raw[0])
     o = 1
     rmh = raw[o:o + maxcy + 1].copy(); o += maxcy + 1

@@ -9,7 +9,7 @@
 !
 ! Usage: mg_ref <control file>
 !   line 1: nbx nby nbz levels lsm maxcy min_cycles nu_pre nu_post
-!   line 2: tol
+!   line 2: tol mg_itrsch (1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS)
 !   line 3: bc_w bc_e bc_s bc_n bc_b bc_t
 !   line 4: x extents of the nbx block columns
 !   line 5: y extents (nby values)
@@ -41,7 +41,7 @@ program mg_ref_driver
   call get_command_argument(1, ctl)
   open(newunit=u, file=trim(ctl), status='old')
   read(u, *) nbx, nby, nbz, ngrid_input, lsmf, maxcy, mincy, irestr, iproln
-  read(u, *) tol
+  read(u, *) tol, mg_itrsch
   read(u, *) bcs
   allocate(ex(nbx), ey(nby), ez(nbz))
   read(u, *) ex
@@ -56,7 +56,6 @@ program mg_ref_driver
   num_domains = nbx * nby * nbz
   nbp = num_domains
   ngrd_gl = ngrid_input
-  mg_itrsch = 3
   L_LSM = (lsmf == 1)
   pl = 1
   g_dx = gdx; g_dy = gdy; g_dz = gdz

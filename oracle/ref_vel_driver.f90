@@ -16,7 +16,8 @@
 !
 ! Usage: mg_ref_vel <control file>
 !   line 1: nbx nby nbz levels lsm maxcy nu_pre nu_post
-!   line 2: eps (absolute: rmax = max|div u| dV, newsolv_mg.for:56)
+!   line 2: eps (absolute: rmax = max|div u| dV, newsolv_mg.for:56) and
+!           mg_itrsch (1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS)
 !   line 3: bc_w bc_e bc_s bc_n bc_b bc_t
 !   lines 4-6: block extents along x, y, z
 !   line 7: dx dy dz
@@ -47,7 +48,7 @@ program mg_ref_vel_driver
   call get_command_argument(1, ctl)
   open(newunit=u, file=trim(ctl), status='old')
   read(u, *) nbx, nby, nbz, ngrid_input, lsmf, maxcy, irestr, iproln
-  read(u, *) eps
+  read(u, *) eps, mg_itrsch
   read(u, *) bcs
   allocate(ex(nbx), ey(nby), ez(nbz))
   read(u, *) ex
@@ -63,7 +64,6 @@ program mg_ref_vel_driver
   num_domains = nbx * nby * nbz
   nbp = num_domains
   ngrd_gl = ngrid_input
-  mg_itrsch = 3
   L_LSM = (lsmf == 1 .or. lsmf == 2)
   pl = 1
   pl_ex = 0

@@ -237,10 +237,12 @@ def calmas(grid, u, v, w, dt, lsm=False):
 
 
 def newsolv_velocity(grid, levels, ustar, vstar, wstar, p0, eps, maxcy, dt, alfapr, lsm=False,
-                     dens=None, coarse_op="reference"):
-    """newsolv_mg.for:12-58 in velocity form.  Returns (iterations, rmax
-    history, p, u, v, w, su) with ghosted block arrays."""
-    o = OracleMG(grid.blocks, levels, lsm=lsm, coarse_op=coarse_op)
+                     dens=None, coarse_op="reference", smoother=3, nu_pre=2, nu_post=2):
+    """newsolv_mg.for:12-58 in velocity form (mg_itrsch = smoother, irestr
+    = nu_pre, iproln = nu_post).  Returns (iterations, rmax history, p, u,
+    v, w, su) with ghosted block arrays."""
+    o = OracleMG(grid.blocks, levels, lsm=lsm, coarse_op=coarse_op, smoother=smoother, nu_pre=nu_pre,
+                 nu_post=nu_post)
     if lsm:
         o.set_density([d[1:-1, 1:-1, 1:-1] for d in dens])
     o.coeff()
