@@ -198,6 +198,63 @@ def run_levelset(grid, levels, precision, steps, warmup):
     return dict(ms=float(np.mean(lat)), lat=lat, launches=launches, kernel_ms=kms, cells=phi.numel(), band=band)
 
 
+def shipped_grid():
+    """The layout of the reference's shipped deck (infodom.cin, control.cin):
+    80 blocks of 40 x 10 x 25 cells lined up in x, dx = 0.005, periodic in
+    x, walls south/north/bottom, slip top."""
+    from multiflow3d_amd import topology as T
+    return T.block_grid([40] * 80, [10], [25], [T.PERIODIC, T.PERIODIC, T.NEUMANN, T.NEUMANN, T.NEUMANN, 3],
+                        [0.005] * 3)
+
+
+def run_shipped(precision, steps, warmup, time_kernel):
+    """newsolv_mg in the velocity form with the shipped deck's multigrid
+    settings (control.cin: ngrid 1, mg_itrsch 2 = ADI-TDMA, maxcy 30,
+    irestr 2, iproln 1) on resident seeded u*, v*, w*, p from 0, eps = 0
+    so every call runs the deck's 30 cycles.  Steps timed one by one on the
+    host around the synchronous call."""
+    import torch
+
+    from multiflow3d_amd import PressureMultigrid
+    from multiflow3d_amd import _lib
+    from multiflow3d_amd import workloads as W
+
+    grid = shipped_grid()
+    dev = torch.device("cuda:0")
+    us, vs, ws = W.velocity_inputs(grid, W.BASE_SEED + 95)
+    init = [torch.from_numpy(np.ascontiguousarray(np.concatenate([a.ravel() for a in x]))).to(dev)
+            for x in (us, vs, ws)]
+    work = [t.clone() for t in init] + [torch.zeros_like(init[0])]
+    torch.cuda.synchronize()
+    mg = PressureMultigrid(grid, 1, nu_pre=2, nu_post=1, smoother=2, maxcy=30, precision=precision)
+
+    def reset():
+        for dst, src in zip(work[:3], init):
+            dst.copy_(src)
+        work[3].zero_()
+        torch.cuda.synchronize()
+
+    ptrs = [t.data_ptr() for t in init] + [t.data_ptr() for t in work]
+    if time_kernel:
+        mg.time_kernel(_lib.MF3D_K_SMOOTH)
+    for _ in range(warmup):
+        reset()
+        mg.newsolv_device(ptrs, 0.0025, 1.0, 0.0, warn=False)
+    if time_kernel:
+        mg.time_kernel(_lib.MF3D_K_SMOOTH)
+    lat, res = [], None
+    for _ in range(steps):
+        reset()
+        ts = time.perf_counter()
+        res = mg.newsolv_device(ptrs, 0.0025, 1.0, 0.0, warn=False)
+        lat.append((time.perf_counter() - ts) * 1e3)
+    launches, kms = mg.kernel_time() if time_kernel else (0, 0.0)
+    mg.time_kernel(0)
+    mg.close()
+    return dict(ms=float(np.mean(lat)), lat=lat, iters=res.iters, rmax=res.rmax, launches=launches, kernel_ms=kms,
+                cells=sum(b.n[0] * b.n[1] * b.n[2] for b in grid.blocks))
+
+
 def run_serving(count):
     """Serving sweep: Poisson arrivals of independent solves, n^3 with n
     uniform in {32,64,96,128,192,256}, tolerance uniform in {1e-6,1e-8,1e-10},
@@ -242,6 +299,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-velocity", action="store_true",
                     help="skip the velocity-form (calvel/calmas) leg")
+    ap.add_argument("--no-shipped", action="store_true",
+                    help="skip the shipped-deck leg (80 blocks, one level, ADI smoother)")
     ap.add_argument("--no-levelset", action="store_true",
                     help="skip the level-set density update (heaviside + coeff) leg")
     ap.add_argument("--serving-requests", type=int, default=64,
@@ -401,6 +460,39 @@ def main():
         levelset["workload"] = (f"heaviside + bound_lsm(18/19) + coeff per time step on the same {n}^3 "
                                 f"grid, {L} levels: phi = signed distance to a sphere r = 0.3, "
                                 f"densl/densg 1000/1.2, mul/mug 1e-3/1.8e-5, dens and mu returned")
+    shipped = None
+    if not args.no_shipped:
+        shipped = {}
+        for prec in ("fp64", "mixed"):
+            r = run_shipped(prec, args.steps, args.warmup, False)
+            t = run_shipped(prec, 2, 1, True)
+            sweep_us = t["kernel_ms"] / t["launches"] * 1e3 if t["launches"] else None
+            shipped[prec] = {"ms_per_call": round(r["ms"], 4), "cycles": r["iters"],
+                             "ms_per_cycle": round(r["ms"] / max(r["iters"], 1), 4),
+                             "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+                             "adi_sweep_us": None if sweep_us is None else round(sweep_us, 2),
+                             "adi_sweeps_timed": t["launches"]}
+            print(f"# shipped deck {prec}: {r['ms']:.3f} ms per newsolv_mg call ({r['iters']} cycles), "
+                  f"ADI sweep {sweep_us} us", file=sys.stderr, flush=True)
+        cells = r["cells"]
+        if not args.no_cpu_baseline:
+            from oracle.oracle import OracleMG
+            from multiflow3d_amd import workloads as W
+            g = shipped_grid()
+            o = OracleMG(g.blocks, 1, nu_pre=2, nu_post=1, smoother=2)
+            o.coeff()
+            f = W.split_blocks(g, W.global_rhs(g, W.BASE_SEED + 96))
+            t0 = time.perf_counter()
+            o.solve(f, 0.0, 30)
+            t_cpu = time.perf_counter() - t0
+            o.close()
+            shipped["cpu_oracle_ms_per_call"] = round(t_cpu * 1e3, 1)
+            shipped["cpu_sample"] = ("the C restatement (oracle/mg_oracle.c), 1 thread, 30 ADI cycles of the "
+                                     "Poisson form on the same layout")
+        shipped["workload"] = (f"the reference's shipped deck layout (80 blocks of 40x10x25 = {cells} cells, "
+                               f"dx 0.005, periodic x, walls S/N/B, slip top) with its multigrid settings: "
+                               f"1 level, ADI-TDMA (mg_itrsch 2), irestr 2, iproln 1, 30 cycles per call, "
+                               f"velocity form, u* seeded, p from 0")
     serving = run_serving(args.serving_requests) if args.serving_requests > 0 else None
 
     lat = np.array(rmx["lat"])
@@ -438,6 +530,7 @@ def main():
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
         "velocity_form": velocity,
         "levelset_update": levelset,
+        "shipped_deck": shipped,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

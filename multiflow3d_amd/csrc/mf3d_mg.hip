@@ -52,6 +52,12 @@ struct Level {
     BlockLevel *db = nullptr;
     Tile *t_sw = nullptr, *t_in = nullptr, *t_row = nullptr, *t_oct = nullptr, *t_ch = nullptr;
     int n_sw = 0, n_in = 0, n_row = 0, n_oct = 0, n_ch = 0;
+    // ADI smoother: dynamic LDS of the whole-sweep kernel (0: the level's
+    // largest block does not fit, the per-pass kernels run), whether the
+    // RHS is staged in LDS too, the largest block's ghosted cells
+    size_t adi_lds = 0;
+    bool adi_rhs = false;
+    int adi_cells = 0;
     int sw_ty = 4;  // sweep tile rows
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
@@ -148,8 +154,8 @@ struct mf3d_mg {
     LsBlock *d_lsb = nullptr;
     long long ls_max_cells = 0;
     double *d_lsio[2] = {nullptr, nullptr};
-    // ADI smoother (mg_itrsch = 2): the Thomas a(m) of the lines in flight,
-    // at the offsets of their cells (level-0 size, shared by all levels)
+    // ADI smoother: a(m) scratch of the per-pass kernels (level-0 size,
+    // shared by the levels whose blocks do not fit in LDS)
     void *d_adi = nullptr;
 };
 
@@ -390,7 +396,8 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     }
     tot += gh0 * 8 * 3;             // p (two buffers) and f
     if (c.krylov == MF3D_OUTER_GCR) tot += gh0 * 8 * 4;  // d (two buffers), w, r
-    if (c.smoother == 2) tot += gh0 * w;                 // ADI's a(m) scratch
+    if (c.smoother == 2) tot += gh0 * w;                 // ADI a(m) scratch (blocks too large for LDS)
+
     tot += std::max(gh0, in0) * 8;  // I/O staging
     return tot;
 }
@@ -480,9 +487,18 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
                            L.t_sw, done);
 }
 
+// Threads of an ADI workgroup: the most lines one diagonal of the level holds.
+unsigned adi_threads(const Level &V) {
+    int lines = 1;  // the most lines one diagonal of any pass holds
+    for (const BlockLevel &g : V.hb)
+        lines = std::max({lines, std::min(g.n0, g.n2), std::min(g.n1, g.n2), std::min(g.n0, g.n1)});
+    return (unsigned)std::min(256, (lines + 63) / 64 * 64);
+}
+
 // One in-place sweep of the lexicographic (1) or ADI (2) smoother on level
-// l's current potential: a launch per hyperplane, or per line diagonal and
-// direction (j, i, k lines, mgsolver.for:433-535).
+// l's current potential: a launch per hyperplane, or one launch per
+// direction (j, i, k lines, mgsolver.for:433-535) with a workgroup per
+// block.
 template <typename T, bool VAR>
 void sweep_inplace(mf3d_mg *mg, Level &L, const int *done) {
     int m0 = 1, m1 = 1, m2 = 1, nb = mg->cfg.nblocks;
@@ -503,23 +519,20 @@ void sweep_inplace(mf3d_mg *mg, Level &L, const int *done) {
                                done);
         return;
     }
-    T *sa = (T *)mg->d_adi;
-    auto pass = [&](auto ax, int nu_max, int nv_max, int lines_max) {
-        constexpr int AX = decltype(ax)::value;
-        const dim3 grid((unsigned)((lines_max + 63) / 64), (unsigned)nb);
-        for (int d = 2; d <= nu_max + nv_max; ++d)
-            hipLaunchKernelGGL((k_adi<T, VAR, AX>), grid, dim3(64), 0, mg->stream, p, sa, rhs, cx, cy, cz, L.db, d,
-                               done);
-    };
-    int lj = 1, li = 1, lk = 1;  // the most lines one diagonal of a pass holds
-    for (const BlockLevel &g : L.hb) {
-        lj = std::max(lj, std::min(g.n0, g.n2));
-        li = std::max(li, std::min(g.n1, g.n2));
-        lk = std::max(lk, std::min(g.n0, g.n1));
+    const unsigned nt = adi_threads(L);
+    if (L.adi_lds) {
+        if (L.adi_rhs)
+            hipLaunchKernelGGL((k_adi_lds<T, VAR, true>), dim3(nb), dim3(nt), L.adi_lds, mg->stream, p, rhs, cx, cy, cz,
+                               L.db, L.adi_cells, done);
+        else
+            hipLaunchKernelGGL((k_adi_lds<T, VAR, false>), dim3(nb), dim3(nt), L.adi_lds, mg->stream, p, rhs, cx, cy,
+                               cz, L.db, L.adi_cells, done);
+        return;
     }
-    pass(std::integral_constant<int, 1>{}, m0, m2, lj);  // j lines, (u, v) = (i, k)
-    pass(std::integral_constant<int, 0>{}, m1, m2, li);  // i lines, (j, k)
-    pass(std::integral_constant<int, 2>{}, m1, m0, lk);  // k lines, (j, i)
+    T *sa = (T *)mg->d_adi;
+    hipLaunchKernelGGL((k_adi<T, VAR, 1>), dim3(nb), dim3(nt), 0, mg->stream, p, sa, rhs, cx, cy, cz, L.db, done);
+    hipLaunchKernelGGL((k_adi<T, VAR, 0>), dim3(nb), dim3(nt), 0, mg->stream, p, sa, rhs, cx, cy, cz, L.db, done);
+    hipLaunchKernelGGL((k_adi<T, VAR, 2>), dim3(nb), dim3(nt), 0, mg->stream, p, sa, rhs, cx, cy, cz, L.db, done);
 }
 
 // `sweeps` sweeps at level l; the first treats the potential as zero when
@@ -916,6 +929,51 @@ int choose_tail(mf3d_mg *mg) {
 template <typename T, bool VAR>
 int autotune_sweeps(mf3d_mg *mg);
 
+// ADI smoother set-up: per level, the whole-sweep LDS kernel where the
+// level's largest block (its ghosted potential and a(m) scratch for every
+// thread, with the RHS if that fits too) fits in kAdiLdsMax bytes.
+constexpr size_t kAdiLdsMax = 150 * 1024;
+
+int adi_setup(mf3d_mg *mg) {
+    const size_t w = inner_bytes(mg->cfg);
+    const char *env = std::getenv("MF3D_ADI_LDS");
+    const bool allow = !(env && env[0] == '0');
+    size_t most = 0;
+    for (Level &V : mg->lv) {
+        size_t cells = 0;
+        int mmax = 1;
+        for (const BlockLevel &g : V.hb) {
+            cells = std::max(cells, (size_t)(g.sz * (g.n2 + 2)));
+            mmax = std::max({mmax, g.n0, g.n1, g.n2});
+        }
+        const size_t scratch = (size_t)adi_threads(V) * (mmax + 1);
+        V.adi_cells = (int)cells;
+        V.adi_lds = 0;
+        V.adi_rhs = false;
+        if (allow && (2 * cells + scratch) * w <= kAdiLdsMax) {
+            V.adi_lds = (2 * cells + scratch) * w;
+            V.adi_rhs = true;
+        } else if (allow && (cells + scratch) * w <= kAdiLdsMax) {
+            V.adi_lds = (cells + scratch) * w;
+        }
+        most = std::max(most, V.adi_lds);
+    }
+    if (most > 64 * 1024) {
+        const hipFuncAttribute attr = hipFuncAttributeMaxDynamicSharedMemorySize;
+        const int bytes = (int)most;
+        const void *fns[8] = {(const void *)k_adi_lds<double, false, false>, (const void *)k_adi_lds<double, false, true>,
+                              (const void *)k_adi_lds<double, true, false>,  (const void *)k_adi_lds<double, true, true>,
+                              (const void *)k_adi_lds<float, false, false>,  (const void *)k_adi_lds<float, false, true>,
+                              (const void *)k_adi_lds<float, true, false>,   (const void *)k_adi_lds<float, true, true>};
+        for (const void *f : fns)
+            if (hipFuncSetAttribute(f, attr, bytes) != hipSuccess) {
+                mg->err = "hipFuncSetAttribute (ADI LDS) failed";
+                return MF3D_EDEVICE;
+            }
+    }
+    return 0;
+}
+
 int build(mf3d_mg *mg) {
     const mf3d_mg_cfg &c = mg->cfg;
     const int L = c.levels, nb = c.nblocks;
@@ -1058,7 +1116,12 @@ int build(mf3d_mg *mg) {
     hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st, mg->cfg.eps);
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
-    if (c.smoother == 2 && (rc = dev_alloc(mg, &mg->d_adi, L0.cells_gh * w))) return rc;
+    if (c.smoother == 2) {
+        if ((rc = adi_setup(mg))) return rc;
+        bool global = false;  // a level runs the per-pass kernels: their a(m) scratch
+        for (const Level &V : mg->lv) global = global || !V.adi_lds;
+        if (global && (rc = dev_alloc(mg, &mg->d_adi, L0.cells_gh * w))) return rc;
+    }
     if (c.smoother != 3) return 0;  // the tiles tuned below belong to the red-black sweep
     if ((rc = mixed ? (c.variable_coef ? autotune_sweeps<float, true>(mg) : autotune_sweeps<float, false>(mg))
                     : (c.variable_coef ? autotune_sweeps<double, true>(mg) : autotune_sweeps<double, false>(mg))))

@@ -16,11 +16,14 @@
 //   neighbours of the lines before it in its pass (new) and after it
 //   (old): with lines indexed (u, v) in loop order, line (u, v) needs
 //   (u-1, v) and (u, v-1) finished and (u+1, v), (u, v+1) untouched --
-//   one launch per diagonal u + v, every line of it one thread.  The
-//   forward recurrence keeps c(m) in the line's own potential entries (no
-//   other line reads them before the launch ends) and a(m) in a scratch
-//   arena at the same offsets; the backward pass then overwrites c with
-//   phi.
+//   the diagonals u + v in increasing order, the lines of a diagonal in
+//   parallel.  Blocks are independent within a sweep (a block reads only
+//   its own ghosts), so one workgroup per block walks the diagonals of a
+//   pass with a barrier between them.  Where the level's blocks fit in
+//   LDS, one launch runs the whole sweep there, the RHS staged in LDS too
+//   when it fits.  The forward recurrence keeps c(m) in the line's own
+//   potential entries (no other line reads them before the diagonal ends)
+//   and a(m) in a scratch; the backward pass overwrites c with phi.
 //
 // The per-cell arithmetic is the reference's: GS as the red-black sweep
 // (accumulated aw*W + ae*E + as*S + an*N + ab*B + at*T, then
@@ -62,71 +65,163 @@ __global__ __launch_bounds__(256) void k_lex(T *__restrict__ p, const T *__restr
     p[q] = RCP ? (r - t) * (T(1) / ap) : (r - t) / ap;
 }
 
-// One diagonal u + v = d of the lines along axis AX (0: i, 1: j, 2: k) of
-// every block; grid: x over the lines of the diagonal in 64s, y = block.
-// Line coordinates in loop order: AX = 1: (u, v) = (i, k); AX = 0:
-// (u, v) = (j, k); AX = 2: (u, v) = (j, i).
+// The cross terms and coefficients of cell (i,j,k) of a line along axis AX:
+// t = rhs - the four cross terms in the reference's order for that pass
+// (mgsolver.for:450-458, 484-492, 520-528), lo / hi the coefficients
+// towards the previous / next cell of the line, ap the diagonal.  The
+// potential is read through P at offset lp and the RHS through R at
+// offset lr (global memory or LDS); the operator at arena offset q.
 template <typename T, bool VAR, int AX>
-__global__ __launch_bounds__(64) void k_adi(T *__restrict__ p, T *__restrict__ sa, const T *__restrict__ rhs,
-                                            const T *__restrict__ cx, const T *__restrict__ cy,
-                                            const T *__restrict__ cz, const BlockLevel *__restrict__ bls, int d,
-                                            const int *__restrict__ done) {
-    if (done && *done) return;
-    const BlockLevel g = bls[blockIdx.y];
-    // extents of the line axis (m) and of the two line coordinates (u, v)
+__device__ __forceinline__ void adi_terms(const BlockLevel &g, const T *P, long long lp, long long sy, long long sz,
+                                          const T *R, long long lr, const T *__restrict__ cx,
+                                          const T *__restrict__ cy, const T *__restrict__ cz, long long q, int i,
+                                          int j, int k, T &t, T &lo, T &hi, T &ap) {
+    T aw, ae, as, an, ab, at;
+    coefs<T, VAR>(g, cx, cy, cz, q, i, j, k, aw, ae, as, an, ab, at, ap);
+    if (AX == 1) {
+        lo = as;
+        hi = an;
+        t = R[lr] - ae * P[lp + 1];
+        t = t - aw * P[lp - 1];
+        t = t - at * P[lp + sz];
+        t = t - ab * P[lp - sz];
+    } else if (AX == 0) {
+        lo = aw;
+        hi = ae;
+        t = R[lr] - an * P[lp + sy];
+        t = t - as * P[lp - sy];
+        t = t - at * P[lp + sz];
+        t = t - ab * P[lp - sz];
+    } else {
+        lo = ab;
+        hi = at;
+        t = R[lr] - an * P[lp + sy];
+        t = t - as * P[lp - sy];
+        t = t - ae * P[lp + 1];
+        t = t - aw * P[lp - 1];
+    }
+}
+
+// The lines of one diagonal u + v = d of a pass (lines along AX) over
+// block g.  LDS: P is the block's ghosted potential in LDS and A a
+// per-thread LDS scratch for a(m) (entry m*blockDim + thread); otherwise P
+// is the level arena and A a level-sized scratch at the cells' offsets.
+// RLDS: R is the block's RHS in LDS, else the arena.  Each line runs in
+// chunks of U cells whose loads are all issued before that part of the
+// recurrence.
+template <typename T, bool VAR, int AX, bool LDS, bool RLDS>
+__device__ __forceinline__ void adi_diagonal(const BlockLevel &g, T *P, const T *R, T *A, const T *__restrict__ cx,
+                                             const T *__restrict__ cy, const T *__restrict__ cz, int d) {
+    constexpr int U = VAR ? 4 : 8;
     const int nm = AX == 0 ? g.n0 : (AX == 1 ? g.n1 : g.n2);
     const int nu = AX == 1 ? g.n0 : g.n1;
     const int nv = AX == 2 ? g.n0 : g.n2;
-    const int u = max(1, d - nv) + (int)(blockIdx.x * 64 + threadIdx.x);
-    const int v = d - u;
-    if (u > nu || u > d - 1 || v < 1) return;
-    const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v);
-    const int j0 = AX == 1 ? 0 : u;
-    const int k0 = AX == 2 ? 0 : v;
-    const long long st = AX == 0 ? 1 : (AX == 1 ? g.sy : g.sz);  // stride along the line
-    long long q = gidx(g, i0, j0, k0);                            // the ghost before the line
-    T a_prev = T(0), c_prev = p[q];
-    for (int m = 1; m <= nm; ++m) {
-        q += st;
-        const int i = AX == 0 ? m : i0, j = AX == 1 ? m : j0, k = AX == 2 ? m : k0;
-        T aw, ae, as, an, ab, at, ap;
-        coefs<T, VAR>(g, cx, cy, cz, q, i, j, k, aw, ae, as, an, ab, at, ap);
-        T lo, hi, t;
-        if (AX == 1) {  // mgsolver.for:450-458
-            lo = as;
-            hi = an;
-            t = rhs[q] - ae * p[q + 1];
-            t = t - aw * p[q - 1];
-            t = t - at * p[q + g.sz];
-            t = t - ab * p[q - g.sz];
-        } else if (AX == 0) {  // :484-492
-            lo = aw;
-            hi = ae;
-            t = rhs[q] - an * p[q + g.sy];
-            t = t - as * p[q - g.sy];
-            t = t - at * p[q + g.sz];
-            t = t - ab * p[q - g.sz];
-        } else {  // :520-528
-            lo = ab;
-            hi = at;
-            t = rhs[q] - an * p[q + g.sy];
-            t = t - as * p[q - g.sy];
-            t = t - ae * p[q + 1];
-            t = t - aw * p[q - 1];
+    const long long sy = g.sy, sz = g.sz;
+    const long long st = AX == 0 ? 1 : (AX == 1 ? sy : sz);  // stride along the line
+    const long long pb = LDS ? 0 : g.off, rb = RLDS ? 0 : g.off;
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    const int ulo = max(1, d - nv), uhi = min(nu, d - 1);
+    for (int u = ulo + tid; u <= uhi; u += nt) {
+        const int v = d - u;
+        const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v);
+        const int j0 = AX == 1 ? 0 : u;
+        const int k0 = AX == 2 ? 0 : v;
+        const long long l0 = i0 + sy * j0 + sz * k0;  // the ghost before the line, block-local
+        auto aix = [&](int m) -> long long { return LDS ? (long long)m * nt + tid : g.off + l0 + st * m; };
+        T a_prev = T(0), c_prev = P[pb + l0];
+        for (int m0 = 1; m0 <= nm; m0 += U) {  // forward: a(m) into A, c(m) into the line's own cells
+            T tt[U], lo[U], hi[U], ap[U];
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                const int m = min(m0 + e, nm);  // past the end: a repeat of the last cell, not used
+                const int i = AX == 0 ? m : i0, j = AX == 1 ? m : j0, k = AX == 2 ? m : k0;
+                const long long l = l0 + st * m;
+                adi_terms<T, VAR, AX>(g, P, pb + l, sy, sz, R, rb + l, cx, cy, cz, g.off + l, i, j, k, tt[e], lo[e],
+                                      hi[e], ap[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                if (m0 + e > nm) break;
+                const T apr = T(1) / (ap[e] - lo[e] * a_prev);
+                a_prev = hi[e] * apr;
+                c_prev = (tt[e] - lo[e] * c_prev) * apr;
+                A[aix(m0 + e)] = a_prev;
+                P[pb + l0 + st * (m0 + e)] = c_prev;
+            }
         }
-        const T apr = T(1) / (ap - lo * a_prev);
-        a_prev = hi * apr;
-        c_prev = (t - lo * c_prev) * apr;
-        sa[q] = a_prev;
-        p[q] = c_prev;
+        T nxt = P[pb + l0 + st * (nm + 1)];  // the ghost after the line
+        for (int m1 = nm; m1 >= 1; m1 -= U) {  // backward: phi(m) = c(m) - a(m)*phi(m+1)
+            T cc[U], aa[U];
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                const int m = max(m1 - e, 1);
+                cc[e] = P[pb + l0 + st * m];
+                aa[e] = A[aix(m)];
+            }
+#pragma unroll
+            for (int e = 0; e < U; ++e) {
+                if (m1 - e < 1) break;
+                nxt = cc[e] - aa[e] * nxt;
+                P[pb + l0 + st * (m1 - e)] = nxt;
+            }
+        }
     }
-    // q is the last cell; phi(m) = c(m) - a(m)*phi(m+1) from the ghost after the line
-    T nxt = p[q + st];
-    for (int m = nm; m >= 1; --m) {
-        nxt = p[q] - sa[q] * nxt;
-        p[q] = nxt;
-        q -= st;
+}
+
+// One pass of ADI (lines along AX) on a level in global memory, one
+// workgroup per block walking the diagonals with a barrier between them;
+// sa is a level-sized scratch for a(m).
+template <typename T, bool VAR, int AX>
+__global__ __launch_bounds__(256) void k_adi(T *__restrict__ p, T *__restrict__ sa, const T *__restrict__ rhs,
+                                             const T *__restrict__ cx, const T *__restrict__ cy,
+                                             const T *__restrict__ cz, const BlockLevel *__restrict__ bls,
+                                             const int *__restrict__ done) {
+    if (done && *done) return;
+    const BlockLevel g = bls[blockIdx.x];
+    const int nu = AX == 1 ? g.n0 : g.n1;
+    const int nv = AX == 2 ? g.n0 : g.n2;
+    for (int d = 2; d <= nu + nv; ++d) {
+        adi_diagonal<T, VAR, AX, false, false>(g, p, rhs, sa, cx, cy, cz, d);
+        __syncthreads();
     }
+}
+
+// A whole ADI sweep (j, i, then k lines) of one block per workgroup with
+// the block in LDS: its ghosted potential, with RLDS its RHS, and the a(m)
+// scratch of every thread (mmax entries each); loaded once, written back
+// once.  Dynamic LDS: (cells + (RLDS ? cells : 0) + blockDim*mmax) entries,
+// cells = the largest block's ghosted box.
+template <typename T, bool VAR, bool RLDS>
+__global__ __launch_bounds__(256) void k_adi_lds(T *__restrict__ p, const T *__restrict__ rhs,
+                                                 const T *__restrict__ cx, const T *__restrict__ cy,
+                                                 const T *__restrict__ cz, const BlockLevel *__restrict__ bls,
+                                                 int cells_max, const int *__restrict__ done) {
+    if (done && *done) return;
+    extern __shared__ __align__(16) unsigned char adi_smem[];
+    T *P = reinterpret_cast<T *>(adi_smem);
+    T *R = P + cells_max;
+    T *A = RLDS ? R + cells_max : R;
+    const BlockLevel g = bls[blockIdx.x];
+    const int cells = (int)(g.sz * (g.n2 + 2));
+    for (int c = (int)threadIdx.x; c < cells; c += (int)blockDim.x) {
+        P[c] = p[g.off + c];
+        if (RLDS) R[c] = rhs[g.off + c];
+    }
+    __syncthreads();
+    const T *Rs = RLDS ? (const T *)R : rhs;
+    for (int d = 2; d <= g.n0 + g.n2; ++d) {
+        adi_diagonal<T, VAR, 1, true, RLDS>(g, P, Rs, A, cx, cy, cz, d);
+        __syncthreads();
+    }
+    for (int d = 2; d <= g.n1 + g.n2; ++d) {
+        adi_diagonal<T, VAR, 0, true, RLDS>(g, P, Rs, A, cx, cy, cz, d);
+        __syncthreads();
+    }
+    for (int d = 2; d <= g.n1 + g.n0; ++d) {
+        adi_diagonal<T, VAR, 2, true, RLDS>(g, P, Rs, A, cx, cy, cz, d);
+        __syncthreads();
+    }
+    for (int c = (int)threadIdx.x; c < cells; c += (int)blockDim.x) p[g.off + c] = P[c];
 }
 
 }  // namespace mf3d

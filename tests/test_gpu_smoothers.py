@@ -107,3 +107,20 @@ def test_shipped_slice_mixed_tracks_fp64():
     np.testing.assert_allclose(out["mixed"][1], out["fp64"][1], rtol=1e-3, atol=0)
     pa, pb = out["mixed"][0], out["fp64"][0]
     assert np.linalg.norm(pa - pb) / np.linalg.norm(pb) < 1e-5
+
+
+@pytest.mark.parametrize("name", ["sm2_multi_uneven_L2", "sm2_lsm_multi_L3", "sm2_shipped_slice_L1"])
+def test_adi_global_memory_path_matches_reference_fixture(name, monkeypatch):
+    """MF3D_ADI_LDS=0 forces the per-pass kernels that work in global
+    memory (the path of blocks too large for LDS): the same bitwise
+    result."""
+    monkeypatch.setenv("MF3D_ADI_LDS", "0")
+    g, m, a = load_sm(name)
+    print(f"[adi global] {name}", flush=True)
+    with _mg(g, m, a, maxcy=m["cycles"], eps=0.0) as mg:
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", NotConverged)
+            p, res = mg.newsolv_mg(a["f"])
+    np.testing.assert_allclose(res.hist, a["rho"], rtol=1e-12, atol=0)
+    for x, e in zip(p, a["p"]):
+        assert np.array_equal(x, e), np.abs(x - e).max()
