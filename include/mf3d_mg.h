@@ -6,6 +6,9 @@
  *                                                     (and mf3d_mg_set_density_device)
  *   subroutine heaviside   lsm.for:1456-1509        -> mf3d_mg_heaviside(_device), which
  *                                                     rebuilds coeff's hierarchy as well
+ *   subroutine tecbin      post.for:496-548         -> mf3d_tecbin_read (the checkpoint the
+ *                                                     reference's restart reads back,
+ *                                                     initial.for:689-760)
  *   subroutine mgkcyc      mgsolver.for:224-369     -> mf3d_mg_vcycle
  *   subroutine newsolv_mg  newsolv_mg.for:2-102     -> mf3d_mg_newsolv (the reference's own
  *                                                       velocity form, with calvel, calmas,
@@ -39,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MF3D_MG_ABI_VERSION 4
+#define MF3D_MG_ABI_VERSION 5
 
 enum {
     MF3D_OK = 0,
@@ -48,7 +51,9 @@ enum {
     MF3D_EINVAL = -22,
     MF3D_ENOTSUP = -95,
     MF3D_ENOMEM = -12,
-    MF3D_EDEVICE = -5
+    MF3D_EDEVICE = -5,
+    MF3D_ENOENT = -2,   /* checkpoint reader: the file cannot be opened */
+    MF3D_EFORMAT = -74  /* checkpoint reader: the file is not in the expected format */
 };
 
 enum { MF3D_FP64 = 0, MF3D_MIXED = 1 };
@@ -253,6 +258,46 @@ int mf3d_mg_set_eps(mf3d_mg *mg, double eps);
 /* 1 if the last solve replayed captured cycle graphs, 0 if it launched its
  * kernels one by one (no counterpart in the reference; for benchmarks). */
 int mf3d_mg_graph_replay(const mf3d_mg *mg);
+
+/* ------------------------------------------------------------------------
+ * The reference's checkpoint, to start the solver from a real run's state.
+ *
+ * tecbin####.bin: one file per block, written by tecbin (post.for:496-548)
+ * and read at restart by initial.for:708-748.  Fortran unformatted
+ * sequential (little-endian, 4-byte record markers): the extents, pl, the
+ * wall flags, then one record per cell (i fastest, then j, then k) of 19
+ * FP64 values: x(i), y(j), z(k) and the 16 fields p, pm, ppm, vis, u, um,
+ * uum, v, vm, vvm, w, wm, wwm, uvm, uwm, vwm. */
+typedef struct {
+    int32_t tt[3];  /* ttc_i, ttc_j, ttc_k: extents with the pl ghost layers */
+    int32_t pl;     /* ghost layers */
+    int32_t ind[3]; /* inind, jnind, knind: -1 where the block ends at a physical
+                       non-periodic east / north / top face, else 0 */
+} mf3d_tecbin_hdr;
+
+/* The header alone. */
+int mf3d_tecbin_header(const char *path, mf3d_tecbin_hdr *hdr);
+
+/* The whole file.  fields (may be NULL): 16 pointers in the order above,
+ * each to ttc_i*ttc_j*ttc_k doubles in the reference's own array layout
+ * (column-major, i fastest); a NULL entry skips that field.  x, y, z (may
+ * be NULL): ttc_i, ttc_j, ttc_k doubles.  hdr (may be NULL) receives the
+ * header.  Returns MF3D_ENOENT or MF3D_EFORMAT (text via
+ * mf3d_io_last_error) on an unreadable or malformed file. */
+int mf3d_tecbin_read(const char *path, mf3d_tecbin_hdr *hdr, double *const *fields, double *x, double *y,
+                     double *z);
+
+/* final_ctime.dat (written flosol.for:296-299, read initial.for:692-694):
+ * format '(i8,3F15.6)' over ntime, ctime, forcn, qstpn, count, ntav1_count,
+ * ntav2_count, i.e. two lines. */
+typedef struct {
+    double ctime, forcn, qstpn, ntav1_count, ntav2_count;
+    int32_t ntime, count;
+} mf3d_final_ctime;
+int mf3d_final_ctime_read(const char *path, mf3d_final_ctime *out);
+
+/* Text of the last checkpoint-reader failure in this thread. */
+const char *mf3d_io_last_error(void);
 
 const char *mf3d_mg_last_error(const mf3d_mg *mg);
 void mf3d_mg_destroy(mf3d_mg *mg);

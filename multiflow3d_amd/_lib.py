@@ -23,8 +23,10 @@ EXPORTS = (
     "mf3d_mg_abi_version", "mf3d_mg_time_kernel", "mf3d_mg_kernel_time",
     "mf3d_mg_graph_replay", "mf3d_mg_set_eps", "mf3d_mg_newsolv", "mf3d_mg_newsolv_device",
     "mf3d_mg_ghosted_cells", "mf3d_mg_set_density_device", "mf3d_mg_heaviside",
-    "mf3d_mg_heaviside_device",
+    "mf3d_mg_heaviside_device", "mf3d_tecbin_header", "mf3d_tecbin_read", "mf3d_final_ctime_read",
+    "mf3d_io_last_error",
 )
+MF3D_ENOENT, MF3D_EFORMAT = -2, -74
 MF3D_COARSE_REFERENCE, MF3D_COARSE_FACE = 0, 1
 MF3D_OUTER_DEFECT, MF3D_OUTER_GCR = 0, 1
 MF3D_K_OFF, MF3D_K_SMOOTH, MF3D_K_OUTER, MF3D_K_RESTRICT, MF3D_K_PROLONG, MF3D_K_HEAVISIDE = 0, 1, 2, 3, 4, 5
@@ -35,6 +37,16 @@ class BlockDesc(ctypes.Structure):
                 ("ld", ctypes.c_int32 * 3), ("h", ctypes.c_double * 3),
                 ("nbr", ctypes.c_int32 * 6), ("per", ctypes.c_int32 * 6),
                 ("bc", ctypes.c_int32 * 6)]
+
+
+class TecbinHdr(ctypes.Structure):
+    _fields_ = [("tt", ctypes.c_int32 * 3), ("pl", ctypes.c_int32), ("ind", ctypes.c_int32 * 3)]
+
+
+class FinalCtime(ctypes.Structure):
+    _fields_ = [("ctime", ctypes.c_double), ("forcn", ctypes.c_double), ("qstpn", ctypes.c_double),
+                ("ntav1_count", ctypes.c_double), ("ntav2_count", ctypes.c_double),
+                ("ntime", ctypes.c_int32), ("count", ctypes.c_int32)]
 
 
 class Cfg(ctypes.Structure):
@@ -108,6 +120,14 @@ def lib():
     L.mf3d_mg_set_density_device.argtypes = [vp, vp, vp]
     L.mf3d_mg_heaviside.restype = ctypes.c_int
     L.mf3d_mg_heaviside.argtypes = [vp, dpp, d, d, d, d, dpp, dpp]
+    L.mf3d_tecbin_header.restype = ctypes.c_int
+    L.mf3d_tecbin_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(TecbinHdr)]
+    L.mf3d_tecbin_read.restype = ctypes.c_int
+    L.mf3d_tecbin_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(TecbinHdr), dpp, vp, vp, vp]
+    L.mf3d_final_ctime_read.restype = ctypes.c_int
+    L.mf3d_final_ctime_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(FinalCtime)]
+    L.mf3d_io_last_error.restype = ctypes.c_char_p
+    L.mf3d_io_last_error.argtypes = []
     L.mf3d_mg_heaviside_device.restype = ctypes.c_int
     L.mf3d_mg_heaviside_device.argtypes = [vp, vp, d, d, d, d, vp, vp, vp]
     _lib = L

@@ -2185,3 +2185,6 @@ void mf3d_mg_destroy(mf3d_mg *mg) {
 }
 
 }  // extern "C"
+
+// host-side reader of the reference's checkpoint files
+#include "mf3d_tecbin.hpp"

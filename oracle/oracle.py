@@ -255,10 +255,10 @@ def have_ref_vel() -> bool:
 
 
 def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None, nu_pre=2, nu_post=2,
-                  smoother=3):
+                  smoother=3, cwd=None):
     """One run of mg_ref_vel in `mode` (0 constant, 1 density, 2 level set,
-    3 bound_lsm only) on the concatenated input `parts`; returns its raw
-    f64 output."""
+    3 bound_lsm only, 4 constant + checkpoint files written into cwd) on
+    the concatenated input `parts`; returns its raw f64 output."""
     if not have_ref_vel():
         raise RuntimeError("compiled reference not built (make -C oracle ref)")
     with tempfile.TemporaryDirectory() as td:
@@ -279,7 +279,7 @@ def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None,
             if mode == 2:
                 fh.write(" ".join(repr(float(x)) for x in props) + "\n")
             fh.write(fin + "\n" + fout + "\n")
-        subprocess.run([REF_VEL_BIN, ctl], check=True, capture_output=True, timeout=600)
+        subprocess.run([REF_VEL_BIN, ctl], check=True, capture_output=True, timeout=600, cwd=cwd)
         return np.fromfile(fout, dtype=np.float64)
 
 
@@ -320,6 +320,38 @@ def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: i
     out, o = _boxes(grid, raw, o, keys)
     res = (k, rmh[:k + 1], out["p"], out["u"], out["v"], out["w"], out["su"])
     return res + (out["dens"], out["mu"]) if mode == 2 else res
+
+
+CHECKPOINT_FIELDS = ("p", "pm", "ppm", "vis", "u", "um", "uum", "v", "vm", "vvm", "w", "wm", "wwm", "uvm", "uwm",
+                     "vwm")
+
+
+def run_ref_checkpoint(grid, levels: int, ustar, vstar, wstar, p0, maxcy: int, dt: float, alfapr: float,
+                       workdir: str):
+    """The compiled reference's velocity-form loop (eps = 0, maxcy cycles)
+    followed by its own checkpoint writer tecbin (post.for:496-548) and
+    final_ctime.dat, both written into workdir.  Returns, per block, the
+    dict of what was checkpointed: x, y, z (1-D) and the 16 fields of
+    CHECKPOINT_FIELDS as ghosted arrays shaped (z+2, y+2, x+2)."""
+    parts = []
+    for b in range(len(grid.blocks)):
+        parts += [ustar[b], vstar[b], wstar[b], p0[b]]
+    raw = _ref_vel_call(grid, levels, 4, parts, maxcy, 0.0, dt, alfapr, cwd=workdir)
+    o = 1 + maxcy + 1
+    _, o = _boxes(grid, raw, o, ("p", "u", "v", "w", "su"))
+    out = []
+    for blk in grid.blocks:
+        nx, ny, nz = blk.n
+        d = {}
+        for key, m in (("x", nx + 2), ("y", ny + 2), ("z", nz + 2)):
+            d[key] = raw[o:o + m].copy()
+            o += m
+        m = (nx + 2) * (ny + 2) * (nz + 2)
+        for key in CHECKPOINT_FIELDS:
+            d[key] = raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy()
+            o += m
+        out.append(d)
+    return out
 
 
 def run_ref_bound_lsm(grid, field):

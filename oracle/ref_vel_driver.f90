@@ -13,6 +13,11 @@
 ! newsolv_mg, as the reference's time step does (flosol.for:114-117), and
 ! writes dens and mu as well.  lsm = 3 only fills the ghosts of a given
 ! cell field through bound_lsm(19) (the viscosity's path) and writes it.
+! lsm = 4 runs like lsm = 0, then writes the reference's checkpoint with
+! its own tecbin (post.for:496-548) -- tecbin####.bin in the working
+! directory, with the mean and viscosity fields filled from the solution --
+! and final_ctime.dat in the reference's format (flosol.for:296-299), and
+! appends to the output the x, y, z and 16 field arrays it checkpointed.
 !
 ! Usage: mg_ref_vel <control file>
 !   line 1: nbx nby nbz levels lsm maxcy nu_pre nu_post
@@ -65,6 +70,7 @@ program mg_ref_vel_driver
   nbp = num_domains
   ngrd_gl = ngrid_input
   L_LSM = (lsmf == 1 .or. lsmf == 2)
+  ntime = 0; ctime = 0.0; forcn = 0.0; qstpn = 0.0; count = 0; ntav1_count = 0.0; ntav2_count = 0.0
   pl = 1
   pl_ex = 0
   differencing = 1
@@ -287,6 +293,59 @@ program mg_ref_vel_driver
       write(u) dom(ib)%mu
     end if
   end do
+  if (lsmf == 4) then
+    do ib = 1, nbp
+      mi = dom(ib)%ttc_i; mj = dom(ib)%ttc_j; mk = dom(ib)%ttc_k
+      allocate(dom(ib)%x(mi), dom(ib)%y(mj), dom(ib)%z(mk))
+      allocate(dom(ib)%pm(mi,mj,mk), dom(ib)%ppm(mi,mj,mk), dom(ib)%vis(mi,mj,mk), &
+               dom(ib)%um(mi,mj,mk), dom(ib)%uum(mi,mj,mk), dom(ib)%vm(mi,mj,mk), &
+               dom(ib)%vvm(mi,mj,mk), dom(ib)%wm(mi,mj,mk), dom(ib)%wwm(mi,mj,mk), &
+               dom(ib)%uvm(mi,mj,mk), dom(ib)%uwm(mi,mj,mk), dom(ib)%vwm(mi,mj,mk))
+      do i = 1, mi
+        dom(ib)%x(i) = (i - 1 - pl + 0.5d0) * gdx + 0.25d0 * dom_id(ib)
+      end do
+      do j = 1, mj
+        dom(ib)%y(j) = (j - 1 - pl + 0.5d0) * gdy
+      end do
+      do k = 1, mk
+        dom(ib)%z(k) = (k - 1 - pl + 0.5d0) * gdz
+      end do
+      dom(ib)%pm = 0.5d0 * dom(ib)%p; dom(ib)%ppm = dom(ib)%p * dom(ib)%p
+      dom(ib)%vis = 1.0d-6 + 1.0d-3 * abs(dom(ib)%u)
+      dom(ib)%um = 0.5d0 * dom(ib)%u; dom(ib)%uum = dom(ib)%u * dom(ib)%u
+      dom(ib)%vm = 0.5d0 * dom(ib)%v; dom(ib)%vvm = dom(ib)%v * dom(ib)%v
+      dom(ib)%wm = 0.5d0 * dom(ib)%w; dom(ib)%wwm = dom(ib)%w * dom(ib)%w
+      dom(ib)%uvm = dom(ib)%u * dom(ib)%v; dom(ib)%uwm = dom(ib)%u * dom(ib)%w
+      dom(ib)%vwm = dom(ib)%v * dom(ib)%w
+    end do
+    call tecbin(nits)
+    ntime = nits; ctime = 0.0123456789d0 * nits; forcn = -1.5d0; qstpn = 2.25d0
+    count = 7; ntav1_count = 3.0d0; ntav2_count = 12345.678901d0
+    open(newunit=uu, file='final_ctime.dat')
+    write(uu, '(i8,3F15.6)') ntime, ctime, forcn, qstpn, count, ntav1_count, ntav2_count
+    close(uu)
+    do ib = 1, nbp
+      write(u) dom(ib)%x
+      write(u) dom(ib)%y
+      write(u) dom(ib)%z
+      write(u) dom(ib)%p
+      write(u) dom(ib)%pm
+      write(u) dom(ib)%ppm
+      write(u) dom(ib)%vis
+      write(u) dom(ib)%u
+      write(u) dom(ib)%um
+      write(u) dom(ib)%uum
+      write(u) dom(ib)%v
+      write(u) dom(ib)%vm
+      write(u) dom(ib)%vvm
+      write(u) dom(ib)%w
+      write(u) dom(ib)%wm
+      write(u) dom(ib)%wwm
+      write(u) dom(ib)%uvm
+      write(u) dom(ib)%uwm
+      write(u) dom(ib)%vwm
+    end do
+  end if
   close(u)
   call MPI_FINALIZE(ierr2)
 

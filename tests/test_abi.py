@@ -21,7 +21,7 @@ HEADER = os.path.join(ROOT, "include", "mf3d_mg.h")
 def _declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(mf3d_mg_\w+)\s*\(", text))
+    return set(re.findall(r"\b(mf3d_\w+)\s*\(", text))
 
 
 def test_library_exports_every_declared_symbol():
@@ -32,9 +32,9 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
     nm = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
                         text=True, check=True).stdout
-    exported = set(re.findall(r"\sT\s(mf3d_mg_\w+)", nm))
+    exported = set(re.findall(r"\sT\s(mf3d_\w+)", nm))
     assert declared <= exported
-    assert L.mf3d_mg_abi_version() == 4
+    assert L.mf3d_mg_abi_version() == 5
 
 
 def test_product_library_does_not_contain_the_oracle():
