@@ -57,7 +57,7 @@ struct Level {
     // RHS is staged in LDS too, the largest block's ghosted cells
     size_t adi_lds = 0;
     bool adi_rhs = false;
-    int adi_cells = 0;
+    int adi_cells = 0, adi_sstride = 0;
     int sw_ty = 4;  // sweep tile rows
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
@@ -520,13 +520,13 @@ void sweep_inplace(mf3d_mg *mg, Level &L, const int *done) {
         return;
     }
     const unsigned nt = adi_threads(L);
-    if (L.adi_lds) {
+    if (L.adi_lds) {  // phase 1 of each diagonal runs on 256 threads
         if (L.adi_rhs)
-            hipLaunchKernelGGL((k_adi_lds<T, VAR, true>), dim3(nb), dim3(nt), L.adi_lds, mg->stream, p, rhs, cx, cy, cz,
-                               L.db, L.adi_cells, done);
+            hipLaunchKernelGGL((k_adi_lds<T, VAR, true>), dim3(nb), dim3(256), L.adi_lds, mg->stream, p, rhs, cx, cy,
+                               cz, L.db, L.adi_cells, L.adi_sstride, done);
         else
-            hipLaunchKernelGGL((k_adi_lds<T, VAR, false>), dim3(nb), dim3(nt), L.adi_lds, mg->stream, p, rhs, cx, cy,
-                               cz, L.db, L.adi_cells, done);
+            hipLaunchKernelGGL((k_adi_lds<T, VAR, false>), dim3(nb), dim3(256), L.adi_lds, mg->stream, p, rhs, cx, cy,
+                               cz, L.db, L.adi_cells, L.adi_sstride, done);
         return;
     }
     T *sa = (T *)mg->d_adi;
@@ -930,8 +930,8 @@ template <typename T, bool VAR>
 int autotune_sweeps(mf3d_mg *mg);
 
 // ADI smoother set-up: per level, the whole-sweep LDS kernel where the
-// level's largest block (its ghosted potential and a(m) scratch for every
-// thread, with the RHS if that fits too) fits in kAdiLdsMax bytes.
+// level's largest block (its ghosted potential and the per-diagonal
+// scratch, with the RHS if that fits too) fits in kAdiLdsMax bytes.
 constexpr size_t kAdiLdsMax = 150 * 1024;
 
 int adi_setup(mf3d_mg *mg) {
@@ -940,21 +940,21 @@ int adi_setup(mf3d_mg *mg) {
     const bool allow = !(env && env[0] == '0');
     size_t most = 0;
     for (Level &V : mg->lv) {
-        size_t cells = 0;
-        int mmax = 1;
+        size_t cells = 0, diag = 1;  // the largest ghosted box; the most cells one diagonal holds
         for (const BlockLevel &g : V.hb) {
             cells = std::max(cells, (size_t)(g.sz * (g.n2 + 2)));
-            mmax = std::max({mmax, g.n0, g.n1, g.n2});
+            const size_t n0 = g.n0, n1 = g.n1, n2 = g.n2;
+            diag = std::max({diag, std::min(n0, n2) * n1, std::min(n1, n2) * n0, std::min(n0, n1) * n2});
         }
-        const size_t scratch = (size_t)adi_threads(V) * (mmax + 1);
         V.adi_cells = (int)cells;
+        V.adi_sstride = (int)diag;
         V.adi_lds = 0;
         V.adi_rhs = false;
-        if (allow && (2 * cells + scratch) * w <= kAdiLdsMax) {
-            V.adi_lds = (2 * cells + scratch) * w;
+        if (allow && (2 * cells + 4 * diag) * w <= kAdiLdsMax) {
+            V.adi_lds = (2 * cells + 4 * diag) * w;
             V.adi_rhs = true;
-        } else if (allow && (cells + scratch) * w <= kAdiLdsMax) {
-            V.adi_lds = (cells + scratch) * w;
+        } else if (allow && (cells + 4 * diag) * w <= kAdiLdsMax) {
+            V.adi_lds = (cells + 4 * diag) * w;
         }
         most = std::max(most, V.adi_lds);
     }

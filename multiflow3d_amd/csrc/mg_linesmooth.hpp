@@ -186,21 +186,109 @@ __global__ __launch_bounds__(256) void k_adi(T *__restrict__ p, T *__restrict__ 
     }
 }
 
+// One diagonal of a pass in LDS, in three phases:
+//   1. every thread: for the cells of all lines of the diagonal, the cross
+//      terms t (into the cell itself in P -- no other line reads it before
+//      the diagonal ends) and the coefficients lo, hi, ap (into the
+//      scratch S, entry (m, line) at m*nl + line);
+//   2. one lane per line: the forward recurrence from LDS, c(m) into P and
+//      a(m) into S's fourth array;
+//   3. the same lane: phi(m) = c(m) - a(m)*phi(m+1), backwards.
+// Phase 1 is parallel over all cells, so the sequential part of a line
+// runs only the recurrence's own operations.
+template <typename T, bool VAR, int AX, bool RLDS>
+__device__ __forceinline__ void adi_diagonal_lds(const BlockLevel &g, T *P, const T *R, T *S, int sstride,
+                                                 const T *__restrict__ cx, const T *__restrict__ cy,
+                                                 const T *__restrict__ cz, int d) {
+    const int nm = AX == 0 ? g.n0 : (AX == 1 ? g.n1 : g.n2);
+    const int nu = AX == 1 ? g.n0 : g.n1;
+    const int nv = AX == 2 ? g.n0 : g.n2;
+    const long long sy = g.sy, sz = g.sz;
+    const long long st = AX == 0 ? 1 : (AX == 1 ? sy : sz);
+    const long long rb = RLDS ? 0 : g.off;
+    const int ulo = max(1, d - nv), uhi = min(nu, d - 1);
+    const int nl = uhi - ulo + 1;
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    T *LO = S, *HI = S + sstride, *AP = S + 2 * sstride, *A = S + 3 * sstride;
+    auto line0 = [&](int u) -> long long {  // the ghost before line u, block-local
+        const int v = d - u;
+        const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v), j0 = AX == 1 ? 0 : u, k0 = AX == 2 ? 0 : v;
+        return i0 + sy * j0 + sz * k0;
+    };
+    for (int e = tid; e < nl * nm; e += nt) {  // phase 1
+        const int ln = e % nl, m = e / nl + 1;
+        const int u = ulo + ln, v = d - u;
+        const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v), j0 = AX == 1 ? 0 : u, k0 = AX == 2 ? 0 : v;
+        const int i = AX == 0 ? m : i0, j = AX == 1 ? m : j0, k = AX == 2 ? m : k0;
+        const long long l = i + sy * j + sz * k;
+        T t, lo, hi, ap;
+        adi_terms<T, VAR, AX>(g, P, l, sy, sz, R, rb + l, cx, cy, cz, g.off + l, i, j, k, t, lo, hi, ap);
+        P[l] = t;
+        LO[e] = lo;
+        HI[e] = hi;
+        AP[e] = ap;
+    }
+    __syncthreads();
+    constexpr int U = 8;  // LDS reads of a chunk issued before its recurrence steps
+    for (int ln = tid; ln < nl; ln += nt) {  // phases 2 and 3
+        const long long l0 = line0(ulo + ln);
+        T a_prev = T(0), c_prev = P[l0];
+        for (int m0 = 1; m0 <= nm; m0 += U) {
+            T lo[U], hi[U], ap[U], tt[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                const int m = min(m0 + q, nm);
+                const int e = (m - 1) * nl + ln;
+                lo[q] = LO[e];
+                hi[q] = HI[e];
+                ap[q] = AP[e];
+                tt[q] = P[l0 + st * m];
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                if (m0 + q > nm) break;
+                const T apr = T(1) / (ap[q] - lo[q] * a_prev);
+                a_prev = hi[q] * apr;
+                c_prev = (tt[q] - lo[q] * c_prev) * apr;
+                P[l0 + st * (m0 + q)] = c_prev;
+                A[(m0 + q - 1) * nl + ln] = a_prev;
+            }
+        }
+        T nxt = P[l0 + st * (nm + 1)];
+        for (int m1 = nm; m1 >= 1; m1 -= U) {
+            T cc[U], aa[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                const int m = max(m1 - q, 1);
+                cc[q] = P[l0 + st * m];
+                aa[q] = A[(m - 1) * nl + ln];
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                if (m1 - q < 1) break;
+                nxt = cc[q] - aa[q] * nxt;
+                P[l0 + st * (m1 - q)] = nxt;
+            }
+        }
+    }
+}
+
 // A whole ADI sweep (j, i, then k lines) of one block per workgroup with
-// the block in LDS: its ghosted potential, with RLDS its RHS, and the a(m)
-// scratch of every thread (mmax entries each); loaded once, written back
-// once.  Dynamic LDS: (cells + (RLDS ? cells : 0) + blockDim*mmax) entries,
-// cells = the largest block's ghosted box.
+// the block in LDS: its ghosted potential, with RLDS its RHS, and the
+// per-diagonal scratch S of 4 arrays of sstride entries (the most cells a
+// diagonal holds); loaded once, written back once.  Dynamic LDS:
+// (cells + (RLDS ? cells : 0) + 4*sstride) entries, cells = the largest
+// block's ghosted box.
 template <typename T, bool VAR, bool RLDS>
 __global__ __launch_bounds__(256) void k_adi_lds(T *__restrict__ p, const T *__restrict__ rhs,
                                                  const T *__restrict__ cx, const T *__restrict__ cy,
                                                  const T *__restrict__ cz, const BlockLevel *__restrict__ bls,
-                                                 int cells_max, const int *__restrict__ done) {
+                                                 int cells_max, int sstride, const int *__restrict__ done) {
     if (done && *done) return;
     extern __shared__ __align__(16) unsigned char adi_smem[];
     T *P = reinterpret_cast<T *>(adi_smem);
     T *R = P + cells_max;
-    T *A = RLDS ? R + cells_max : R;
+    T *S = RLDS ? R + cells_max : R;
     const BlockLevel g = bls[blockIdx.x];
     const int cells = (int)(g.sz * (g.n2 + 2));
     for (int c = (int)threadIdx.x; c < cells; c += (int)blockDim.x) {
@@ -210,15 +298,15 @@ __global__ __launch_bounds__(256) void k_adi_lds(T *__restrict__ p, const T *__r
     __syncthreads();
     const T *Rs = RLDS ? (const T *)R : rhs;
     for (int d = 2; d <= g.n0 + g.n2; ++d) {
-        adi_diagonal<T, VAR, 1, true, RLDS>(g, P, Rs, A, cx, cy, cz, d);
+        adi_diagonal_lds<T, VAR, 1, RLDS>(g, P, Rs, S, sstride, cx, cy, cz, d);
         __syncthreads();
     }
     for (int d = 2; d <= g.n1 + g.n2; ++d) {
-        adi_diagonal<T, VAR, 0, true, RLDS>(g, P, Rs, A, cx, cy, cz, d);
+        adi_diagonal_lds<T, VAR, 0, RLDS>(g, P, Rs, S, sstride, cx, cy, cz, d);
         __syncthreads();
     }
     for (int d = 2; d <= g.n1 + g.n0; ++d) {
-        adi_diagonal<T, VAR, 2, true, RLDS>(g, P, Rs, A, cx, cy, cz, d);
+        adi_diagonal_lds<T, VAR, 2, RLDS>(g, P, Rs, S, sstride, cx, cy, cz, d);
         __syncthreads();
     }
     for (int c = (int)threadIdx.x; c < cells; c += (int)blockDim.x) p[g.off + c] = P[c];
