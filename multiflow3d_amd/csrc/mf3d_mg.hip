@@ -58,6 +58,7 @@ struct Level {
     size_t adi_lds = 0;
     bool adi_rhs = false;
     int adi_cells = 0, adi_sstride = 0;
+    void *adi_apr[3] = {}, *adi_a[3] = {};  // the LDS kernel's Thomas coefficients per line axis
     int sw_ty = 4;  // sweep tile rows
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
@@ -388,6 +389,7 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
             gh += (long long)(dim_at(b[i], 0, l) + 2) * (dim_at(b[i], 1, l) + 2) * (dim_at(b[i], 2, l) + 2);
         tot += gh * w * 3;                                     // phi (two) and rhs
         if (c.variable_coef) tot += gh * (w * 3 + 8 * 4);      // coefficients, FP64 copies, density
+        if (c.smoother == 2) tot += gh * w * 6;                // ADI Thomas coefficients (LDS levels)
     }
     long long gh0 = 0, in0 = 0;
     for (int i = 0; i < c.nblocks; ++i) {
@@ -520,13 +522,15 @@ void sweep_inplace(mf3d_mg *mg, Level &L, const int *done) {
         return;
     }
     const unsigned nt = adi_threads(L);
-    if (L.adi_lds) {  // phase 1 of each diagonal runs on 256 threads
+    if (L.adi_lds) {  // a 256-thread workgroup per block
+        const AdiCoef<T> ac{{(const T *)L.adi_apr[0], (const T *)L.adi_apr[1], (const T *)L.adi_apr[2]},
+                            {(const T *)L.adi_a[0], (const T *)L.adi_a[1], (const T *)L.adi_a[2]}};
         if (L.adi_rhs)
             hipLaunchKernelGGL((k_adi_lds<T, VAR, true>), dim3(nb), dim3(256), L.adi_lds, mg->stream, p, rhs, cx, cy,
-                               cz, L.db, L.adi_cells, L.adi_sstride, done);
+                               cz, ac, L.db, L.adi_cells, L.adi_sstride, done);
         else
             hipLaunchKernelGGL((k_adi_lds<T, VAR, false>), dim3(nb), dim3(256), L.adi_lds, mg->stream, p, rhs, cx, cy,
-                               cz, L.db, L.adi_cells, L.adi_sstride, done);
+                               cz, ac, L.db, L.adi_cells, L.adi_sstride, done);
         return;
     }
     T *sa = (T *)mg->d_adi;
@@ -950,11 +954,12 @@ int adi_setup(mf3d_mg *mg) {
         V.adi_sstride = (int)diag;
         V.adi_lds = 0;
         V.adi_rhs = false;
-        if (allow && (2 * cells + 4 * diag) * w <= kAdiLdsMax) {
-            V.adi_lds = (2 * cells + 4 * diag) * w;
+        const bool fits = allow && diag <= 1024;  // the next diagonal's prefetch: 4 entries per thread
+        if (fits && (2 * cells + 5 * diag) * w <= kAdiLdsMax) {
+            V.adi_lds = (2 * cells + 5 * diag) * w;
             V.adi_rhs = true;
-        } else if (allow && (cells + 4 * diag) * w <= kAdiLdsMax) {
-            V.adi_lds = (cells + 4 * diag) * w;
+        } else if (fits && (cells + 5 * diag) * w <= kAdiLdsMax) {
+            V.adi_lds = (cells + 5 * diag) * w;
         }
         most = std::max(most, V.adi_lds);
     }
@@ -971,6 +976,28 @@ int adi_setup(mf3d_mg *mg) {
                 return MF3D_EDEVICE;
             }
     }
+    return 0;
+}
+
+// The ADI Thomas coefficients of the levels whose sweeps run in LDS, from
+// the (inner-precision) operator; after every coefficient build.
+template <typename T, bool VAR>
+int adi_coef_build(mf3d_mg *mg) {
+    for (Level &V : mg->lv) {
+        if (!V.adi_lds) continue;
+        int most = 1;
+        for (const BlockLevel &g : V.hb) most = std::max({most, g.n0 * g.n2, g.n1 * g.n2, g.n0 * g.n1});
+        const dim3 grid((unsigned)((most + 255) / 256), (unsigned)mg->cfg.nblocks);
+        const T *cx = (const T *)V.cx, *cy = (const T *)V.cy, *cz = (const T *)V.cz;
+        hipLaunchKernelGGL((k_adi_coef<T, VAR, 0>), grid, dim3(256), 0, mg->stream, (T *)V.adi_apr[0], (T *)V.adi_a[0],
+                           cx, cy, cz, V.db);
+        hipLaunchKernelGGL((k_adi_coef<T, VAR, 1>), grid, dim3(256), 0, mg->stream, (T *)V.adi_apr[1], (T *)V.adi_a[1],
+                           cx, cy, cz, V.db);
+        hipLaunchKernelGGL((k_adi_coef<T, VAR, 2>), grid, dim3(256), 0, mg->stream, (T *)V.adi_apr[2], (T *)V.adi_a[2],
+                           cx, cy, cz, V.db);
+    }
+    HIPCHK(hipStreamSynchronize(mg->stream));
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -1119,8 +1146,18 @@ int build(mf3d_mg *mg) {
     if (c.smoother == 2) {
         if ((rc = adi_setup(mg))) return rc;
         bool global = false;  // a level runs the per-pass kernels: their a(m) scratch
-        for (const Level &V : mg->lv) global = global || !V.adi_lds;
+        for (Level &V : mg->lv) {
+            global = global || !V.adi_lds;
+            if (V.adi_lds)
+                for (int a = 0; a < 3; ++a)
+                    if ((rc = dev_alloc(mg, &V.adi_apr[a], V.cells_gh * w)) ||
+                        (rc = dev_alloc(mg, &V.adi_a[a], V.cells_gh * w)))
+                        return rc;
+        }
         if (global && (rc = dev_alloc(mg, &mg->d_adi, L0.cells_gh * w))) return rc;
+        // constant coefficients are final now; variable ones at each coeff
+        if (!c.variable_coef && (rc = mixed ? adi_coef_build<float, false>(mg) : adi_coef_build<double, false>(mg)))
+            return rc;
     }
     if (c.smoother != 3) return 0;  // the tiles tuned below belong to the red-black sweep
     if ((rc = mixed ? (c.variable_coef ? autotune_sweeps<float, true>(mg) : autotune_sweeps<float, false>(mg))
@@ -1258,6 +1295,11 @@ int build_lsm(mf3d_mg *mg) {
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
     mg->density_set = true;
+    if (mg->cfg.smoother == 2) {  // the ADI Thomas coefficients follow the operator
+        const int rc = mg->cfg.precision == MF3D_MIXED ? adi_coef_build<float, true>(mg)
+                                                       : adi_coef_build<double, true>(mg);
+        if (rc) return rc;
+    }
     return 0;
 }
 

@@ -186,109 +186,226 @@ __global__ __launch_bounds__(256) void k_adi(T *__restrict__ p, T *__restrict__ 
     }
 }
 
-// One diagonal of a pass in LDS, in three phases:
-//   1. every thread: for the cells of all lines of the diagonal, the cross
-//      terms t (into the cell itself in P -- no other line reads it before
-//      the diagonal ends) and the coefficients lo, hi, ap (into the
-//      scratch S, entry (m, line) at m*nl + line);
-//   2. one lane per line: the forward recurrence from LDS, c(m) into P and
-//      a(m) into S's fourth array;
-//   3. the same lane: phi(m) = c(m) - a(m)*phi(m+1), backwards.
-// Phase 1 is parallel over all cells, so the sequential part of a line
-// runs only the recurrence's own operations.
-template <typename T, bool VAR, int AX, bool RLDS>
-__device__ __forceinline__ void adi_diagonal_lds(const BlockLevel &g, T *P, const T *R, T *S, int sstride,
-                                                 const T *__restrict__ cx, const T *__restrict__ cy,
-                                                 const T *__restrict__ cz, int d) {
+// The Thomas coefficients of every line along AX: apr(m) =
+// 1/(ap - lo*a(m-1)), a(m) = hi*apr(m), a(0) = 0 -- the reference's
+// operations (mgsolver.for:447-449, 481-483, 517-519).  They depend on the
+// operator only, so they are built once per coefficient build and the
+// sweeps' recurrence keeps no division.  Grid: x over the block's lines in
+// 256s, y = block.
+template <typename T, bool VAR, int AX>
+__global__ __launch_bounds__(256) void k_adi_coef(T *__restrict__ apr_out, T *__restrict__ a_out,
+                                                  const T *__restrict__ cx, const T *__restrict__ cy,
+                                                  const T *__restrict__ cz, const BlockLevel *__restrict__ bls) {
+    const BlockLevel g = bls[blockIdx.y];
     const int nm = AX == 0 ? g.n0 : (AX == 1 ? g.n1 : g.n2);
     const int nu = AX == 1 ? g.n0 : g.n1;
     const int nv = AX == 2 ? g.n0 : g.n2;
-    const long long sy = g.sy, sz = g.sz;
-    const long long st = AX == 0 ? 1 : (AX == 1 ? sy : sz);
-    const long long rb = RLDS ? 0 : g.off;
-    const int ulo = max(1, d - nv), uhi = min(nu, d - 1);
-    const int nl = uhi - ulo + 1;
-    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
-    T *LO = S, *HI = S + sstride, *AP = S + 2 * sstride, *A = S + 3 * sstride;
-    auto line0 = [&](int u) -> long long {  // the ghost before line u, block-local
-        const int v = d - u;
-        const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v), j0 = AX == 1 ? 0 : u, k0 = AX == 2 ? 0 : v;
-        return i0 + sy * j0 + sz * k0;
-    };
-    for (int e = tid; e < nl * nm; e += nt) {  // phase 1
-        const int ln = e % nl, m = e / nl + 1;
-        const int u = ulo + ln, v = d - u;
-        const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v), j0 = AX == 1 ? 0 : u, k0 = AX == 2 ? 0 : v;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nu * nv) return;
+    const int u = t % nu + 1, v = t / nu + 1;
+    const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v), j0 = AX == 1 ? 0 : u, k0 = AX == 2 ? 0 : v;
+    const long long st = AX == 0 ? 1 : (AX == 1 ? g.sy : g.sz);
+    const long long q0 = gidx(g, i0, j0, k0);
+    T a_prev = T(0);
+    for (int m = 1; m <= nm; ++m) {
+        const long long q = q0 + st * m;
         const int i = AX == 0 ? m : i0, j = AX == 1 ? m : j0, k = AX == 2 ? m : k0;
-        const long long l = i + sy * j + sz * k;
-        T t, lo, hi, ap;
-        adi_terms<T, VAR, AX>(g, P, l, sy, sz, R, rb + l, cx, cy, cz, g.off + l, i, j, k, t, lo, hi, ap);
-        P[l] = t;
-        LO[e] = lo;
-        HI[e] = hi;
-        AP[e] = ap;
-    }
-    __syncthreads();
-    constexpr int U = 8;  // LDS reads of a chunk issued before its recurrence steps
-    for (int ln = tid; ln < nl; ln += nt) {  // phases 2 and 3
-        const long long l0 = line0(ulo + ln);
-        T a_prev = T(0), c_prev = P[l0];
-        for (int m0 = 1; m0 <= nm; m0 += U) {
-            T lo[U], hi[U], ap[U], tt[U];
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                const int m = min(m0 + q, nm);
-                const int e = (m - 1) * nl + ln;
-                lo[q] = LO[e];
-                hi[q] = HI[e];
-                ap[q] = AP[e];
-                tt[q] = P[l0 + st * m];
-            }
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                if (m0 + q > nm) break;
-                const T apr = T(1) / (ap[q] - lo[q] * a_prev);
-                a_prev = hi[q] * apr;
-                c_prev = (tt[q] - lo[q] * c_prev) * apr;
-                P[l0 + st * (m0 + q)] = c_prev;
-                A[(m0 + q - 1) * nl + ln] = a_prev;
-            }
-        }
-        T nxt = P[l0 + st * (nm + 1)];
-        for (int m1 = nm; m1 >= 1; m1 -= U) {
-            T cc[U], aa[U];
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                const int m = max(m1 - q, 1);
-                cc[q] = P[l0 + st * m];
-                aa[q] = A[(m - 1) * nl + ln];
-            }
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                if (m1 - q < 1) break;
-                nxt = cc[q] - aa[q] * nxt;
-                P[l0 + st * (m1 - q)] = nxt;
-            }
-        }
+        T aw, ae, as, an, ab, at, ap;
+        coefs<T, VAR>(g, cx, cy, cz, q, i, j, k, aw, ae, as, an, ab, at, ap);
+        const T lo = AX == 0 ? aw : (AX == 1 ? as : ab), hi = AX == 0 ? ae : (AX == 1 ? an : at);
+        const T apr = T(1) / (ap - lo * a_prev);
+        a_prev = hi * apr;
+        apr_out[q] = apr;
+        a_out[q] = a_prev;
     }
 }
 
-// A whole ADI sweep (j, i, then k lines) of one block per workgroup with
-// the block in LDS: its ghosted potential, with RLDS its RHS, and the
-// per-diagonal scratch S of 4 arrays of sstride entries (the most cells a
-// diagonal holds); loaded once, written back once.  Dynamic LDS:
-// (cells + (RLDS ? cells : 0) + 4*sstride) entries, cells = the largest
-// block's ghosted box.
+// Pointers to a level's ADI line coefficients, apr and a per line axis.
+template <typename T>
+struct AdiCoef {
+    const T *apr[3];
+    const T *a[3];
+};
+
+// Geometry of one diagonal u + v = d of a pass over block g.
+template <int AX>
+struct AdiDiag {
+    int nm, ulo, nl;
+    long long st;
+    __device__ AdiDiag(const BlockLevel &g, int d) {
+        nm = AX == 0 ? g.n0 : (AX == 1 ? g.n1 : g.n2);
+        const int nu = AX == 1 ? g.n0 : g.n1, nv = AX == 2 ? g.n0 : g.n2;
+        ulo = max(1, d - nv);
+        nl = max(0, min(nu, d - 1) - ulo + 1);
+        st = AX == 0 ? 1 : (AX == 1 ? g.sy : g.sz);
+    }
+    // block-local offset of cell m of the diagonal's line ln (m = 0: the
+    // ghost before the line), and its (i, j, k)
+    __device__ long long cell(const BlockLevel &g, int d, int ln, int m, int &i, int &j, int &k) const {
+        const int u = ulo + ln, v = d - u;
+        const int i0 = AX == 1 ? u : (AX == 0 ? 0 : v), j0 = AX == 1 ? 0 : u, k0 = AX == 2 ? 0 : v;
+        i = AX == 0 ? m : i0;
+        j = AX == 1 ? m : j0;
+        k = AX == 2 ? m : k0;
+        return i + g.sy * j + g.sz * k;
+    }
+    // entry e = (m-1)*nl + ln of the diagonal's cell list -> (ln, m), without
+    // an integer division: a float estimate (e < 2^20, nl <= 1024) corrected
+    // by one step either way
+    __device__ void split(int e, float rnl, int &ln, int &m) const {
+        int q = (int)((float)e * rnl);
+        q -= q * nl > e ? 1 : 0;
+        q += (q + 1) * nl <= e ? 1 : 0;
+        ln = e - q * nl;
+        m = q + 1;
+    }
+};
+
+// The apr and a of diagonal d's cells (entry e = (m-1)*nl + line), up to
+// two per thread of a 256-thread workgroup: loaded into registers now,
+// stored into LDS later, so the loads fly while the previous diagonal runs.
+template <typename T, int AX>
+struct AdiPrefetch {
+    static constexpr int K = 4;  // entries per thread (the scratch holds at most 256*K)
+    T ap[K], aa[K];
+    __device__ void load(const BlockLevel &g, const AdiCoef<T> &ac, int d, int dmax) {
+        if (d > dmax) return;
+        const AdiDiag<AX> D(g, d);
+        const float rnl = 1.0f / (float)max(D.nl, 1);
+        const T *apr = ac.apr[AX] + g.off, *a = ac.a[AX] + g.off;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int e = (int)threadIdx.x + q * 256;
+            if (e < D.nl * D.nm) {
+                int i, j, k, ln, m;
+                D.split(e, rnl, ln, m);
+                const int l = (int)D.cell(g, d, ln, m, i, j, k);
+                ap[q] = apr[l];
+                aa[q] = a[l];
+            }
+        }
+    }
+    __device__ void store(const BlockLevel &g, int d, int dmax, T *APR, T *A) const {
+        if (d > dmax) return;
+        const AdiDiag<AX> D(g, d);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int e = (int)threadIdx.x + q * 256;
+            if (e < D.nl * D.nm) {
+                APR[e] = ap[q];
+                A[e] = aa[q];
+            }
+        }
+    }
+};
+
+// One pass (lines along AX) of a block in LDS.  Per diagonal:
+//   1. every thread: the cross terms t of the diagonal's cells (into the
+//      cells themselves in P -- no other line reads them before the
+//      diagonal ends) and lo (into LO); and the loads of the next
+//      diagonal's apr and a;
+//   2. one lane per line: c(m) = (t - lo*c(m-1))*apr(m) into P;
+//   3. the same lane: phi(m) = c(m) - a(m)*phi(m+1), backwards;
+//   then the next diagonal's apr and a go to the other LDS buffer.
+template <typename T, bool VAR, int AX, bool RLDS>
+__device__ __forceinline__ void adi_pass_lds(const BlockLevel &g, T *P, const T *R, T *LO, T *APR0, T *A0,
+                                             T *APR1, T *A1, const AdiCoef<T> &ac, const T *__restrict__ cx,
+                                             const T *__restrict__ cy, const T *__restrict__ cz) {
+    const int nu = AX == 1 ? g.n0 : g.n1, nv = AX == 2 ? g.n0 : g.n2;
+    const int dmax = nu + nv;
+    const long long rb = RLDS ? 0 : g.off;
+    const int tid = (int)threadIdx.x;
+    AdiPrefetch<T, AX> pf;
+    pf.load(g, ac, 2, dmax);
+    pf.store(g, 2, dmax, APR0, A0);
+    __syncthreads();
+    for (int d = 2; d <= dmax; ++d) {
+        T *APR = (d & 1) ? APR1 : APR0, *A = (d & 1) ? A1 : A0;
+        const AdiDiag<AX> D(g, d);
+        const float rnl = 1.0f / (float)max(D.nl, 1);
+        for (int e = tid; e < D.nl * D.nm; e += 256) {  // phase 1
+            int i, j, k, ln, m;
+            D.split(e, rnl, ln, m);
+            const int l = (int)D.cell(g, d, ln, m, i, j, k);
+            T t, lo, hi, ap;
+            adi_terms<T, VAR, AX>(g, P, l, g.sy, g.sz, R, rb + l, cx, cy, cz, g.off + l, i, j, k, t, lo, hi, ap);
+            P[l] = t;
+            LO[e] = lo;
+        }
+        pf.load(g, ac, d + 1, dmax);
+        __syncthreads();
+        constexpr int U = 8;  // full chunks run without branches, the remainder one cell at a time
+        for (int ln = tid; ln < D.nl; ln += 256) {  // phases 2 and 3
+            int i, j, k;
+            const int l0 = (int)D.cell(g, d, ln, 0, i, j, k);  // LDS offsets fit in 32 bits
+            const int st = (int)D.st, nm = D.nm, nl = D.nl;
+            T c_prev = P[l0];
+            int m = 1;
+            for (; m + U - 1 <= nm; m += U) {
+                const int l = l0 + st * m, e = (m - 1) * nl + ln;
+                T lo[U], ap[U], tt[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    lo[q] = LO[e + q * nl];
+                    ap[q] = APR[e + q * nl];
+                    tt[q] = P[l + q * st];
+                }
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    c_prev = (tt[q] - lo[q] * c_prev) * ap[q];
+                    P[l + q * st] = c_prev;
+                }
+            }
+            for (; m <= nm; ++m) {
+                const int e = (m - 1) * nl + ln;
+                c_prev = (P[l0 + st * m] - LO[e] * c_prev) * APR[e];
+                P[l0 + st * m] = c_prev;
+            }
+            T nxt = P[l0 + st * (nm + 1)];
+            m = nm;
+            for (; m - U + 1 >= 1; m -= U) {
+                const int l = l0 + st * m, e = (m - 1) * nl + ln;
+                T cc[U], aa[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    cc[q] = P[l - q * st];
+                    aa[q] = A[e - q * nl];
+                }
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    nxt = cc[q] - aa[q] * nxt;
+                    P[l - q * st] = nxt;
+                }
+            }
+            for (; m >= 1; --m) {
+                nxt = P[l0 + st * m] - A[(m - 1) * nl + ln] * nxt;
+                P[l0 + st * m] = nxt;
+            }
+        }
+        pf.store(g, d + 1, dmax, (d & 1) ? APR0 : APR1, (d & 1) ? A0 : A1);
+        __syncthreads();
+    }
+}
+
+// A whole ADI sweep (j, i, then k lines) of one block per 256-thread
+// workgroup with the block in LDS: its ghosted potential, with RLDS its
+// RHS, and per-diagonal scratch of 5 arrays of sstride entries (lo, and
+// two buffers of apr and a); loaded once, written back once.  Dynamic
+// LDS: (cells + (RLDS ? cells : 0) + 5*sstride) entries, cells = the
+// largest block's ghosted box; sstride <= 1024 (the prefetch holds 4
+// entries per thread).
 template <typename T, bool VAR, bool RLDS>
 __global__ __launch_bounds__(256) void k_adi_lds(T *__restrict__ p, const T *__restrict__ rhs,
                                                  const T *__restrict__ cx, const T *__restrict__ cy,
-                                                 const T *__restrict__ cz, const BlockLevel *__restrict__ bls,
-                                                 int cells_max, int sstride, const int *__restrict__ done) {
+                                                 const T *__restrict__ cz, AdiCoef<T> ac,
+                                                 const BlockLevel *__restrict__ bls, int cells_max, int sstride,
+                                                 const int *__restrict__ done) {
     if (done && *done) return;
     extern __shared__ __align__(16) unsigned char adi_smem[];
     T *P = reinterpret_cast<T *>(adi_smem);
     T *R = P + cells_max;
-    T *S = RLDS ? R + cells_max : R;
+    T *LO = RLDS ? R + cells_max : R;
+    T *APR0 = LO + sstride, *A0 = APR0 + sstride, *APR1 = A0 + sstride, *A1 = APR1 + sstride;
     const BlockLevel g = bls[blockIdx.x];
     const int cells = (int)(g.sz * (g.n2 + 2));
     for (int c = (int)threadIdx.x; c < cells; c += (int)blockDim.x) {
@@ -297,18 +414,9 @@ __global__ __launch_bounds__(256) void k_adi_lds(T *__restrict__ p, const T *__r
     }
     __syncthreads();
     const T *Rs = RLDS ? (const T *)R : rhs;
-    for (int d = 2; d <= g.n0 + g.n2; ++d) {
-        adi_diagonal_lds<T, VAR, 1, RLDS>(g, P, Rs, S, sstride, cx, cy, cz, d);
-        __syncthreads();
-    }
-    for (int d = 2; d <= g.n1 + g.n2; ++d) {
-        adi_diagonal_lds<T, VAR, 0, RLDS>(g, P, Rs, S, sstride, cx, cy, cz, d);
-        __syncthreads();
-    }
-    for (int d = 2; d <= g.n1 + g.n0; ++d) {
-        adi_diagonal_lds<T, VAR, 2, RLDS>(g, P, Rs, S, sstride, cx, cy, cz, d);
-        __syncthreads();
-    }
+    adi_pass_lds<T, VAR, 1, RLDS>(g, P, Rs, LO, APR0, A0, APR1, A1, ac, cx, cy, cz);
+    adi_pass_lds<T, VAR, 0, RLDS>(g, P, Rs, LO, APR0, A0, APR1, A1, ac, cx, cy, cz);
+    adi_pass_lds<T, VAR, 2, RLDS>(g, P, Rs, LO, APR0, A0, APR1, A1, ac, cx, cy, cz);
     for (int c = (int)threadIdx.x; c < cells; c += (int)blockDim.x) p[g.off + c] = P[c];
 }
 
