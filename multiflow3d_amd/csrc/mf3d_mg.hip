@@ -1167,7 +1167,7 @@ int build(mf3d_mg *mg) {
 }
 
 // Tile autotuner: for every level the sweep kernel runs on, time the sweep
-// over chunk depths tz = 2 .. 64 (the finest level with 8-row tiles, the
+// over chunk depths tz from 2 to 64 (the finest level with 8-row tiles, the
 // others with 4-row tiles) and keep the fastest.  Each candidate is launched
 // once to warm up and then timed over 3 launches with HIP events on the
 // handle's stream, on the level's own (zeroed) arrays; the fields are zeroed
@@ -1216,7 +1216,11 @@ int autotune_sweeps(mf3d_mg *mg) {
         int best_tz = 0;
         std::vector<Tile> best_t;
         Tile *tmp = nullptr;
-        for (int tz = 2; tz <= 64 && tz <= std::max(2, nzmax); tz *= 2) {
+        // powers of two, and depths whose step count (tz + 2) is a whole
+        // number of the sweep's 10-step unrolled blocks
+        static const int kTz[] = {2, 4, 8, 16, 18, 28, 32, 38, 48, 58, 64};
+        for (int tz : kTz) {
+            if (tz > std::max(2, nzmax)) break;
             auto t = xcd_order(sweep_tiles(V.hb, V.sw_ty, tz));
             if (hipMalloc((void **)&tmp, sizeof(Tile) * t.size()) != hipSuccess) break;
             Tile *keep = V.t_sw;

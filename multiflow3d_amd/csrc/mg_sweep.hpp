@@ -25,16 +25,21 @@ namespace mf3d {
 // values the reference computes once.
 //
 // The kernel is bound by instruction issue before HBM, so the march keeps
-// scalar and address work out of the loop.  The main loop is unrolled by
-// 10 (the ring has 5 slots and colours alternate), which makes every LDS
-// slot, colour parity and register set compile-time; the leftover steps
-// (an even number) run first with slots computed at run time.  Global
-// traffic goes through buffer resources: a per-thread 32-bit offset and a
-// per-plane scalar offset, no address arithmetic per load; loads are
-// unconditional (clamped in-bounds offsets) and never masked, since values
-// outside the block feed no update.  Two register sets alternate, so plane
-// c+2 and the next RHS values are in flight while step c computes.  Where
-// no Neumann-zeroed coefficient is in reach of the tile the constant
+// scalar and address work out of the loop.  The loop is unrolled by 10
+// (the ring has 5 slots and colours alternate), which makes every LDS
+// slot, colour parity and register set compile-time.  A chunk whose step
+// count is not a multiple of 10 starts up to 8 steps early: those steps
+// load clamped planes and compute first-colour cells that no stored cell
+// reads (their slots are overwritten by whole planes before use), and
+// store nothing.  Global traffic goes through buffer resources: a
+// per-thread 32-bit offset and a per-plane scalar offset, no address
+// arithmetic per load; loads are unconditional (clamped in-bounds offsets)
+// and never masked, since values outside the block feed no update.  NR
+// register sets rotate, so the loads for step c+NR are in flight while
+// step c computes: 2 for a full sweep (four loads a step keep the queue
+// deep), 5 for the zero-start sweep, which loads only the two RHS values
+// and would otherwise wait on the stores of the steps before.  Where no
+// Neumann-zeroed coefficient is in reach of the tile the constant
 // coefficients and ap are hoisted.
 // ZERO: pin is identically zero (the first sweep after restriction, where
 // the reference has just zeroed phi, mgsolver.for:650-657).
@@ -187,7 +192,8 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
     auto load_regs = [&](Regs &R, int c, auto parc) {
         constexpr int PR = decltype(parc)::value;
         if (!ZERO) {
-            const uint32_t so = (uint32_t)(c + 1 < zhi ? c + 1 : zhi) * pb;
+            const int zc = c + 1 < zhi ? c + 1 : zhi;
+            const uint32_t so = (uint32_t)(zc < 0 ? 0 : zc) * pb;
 #pragma unroll
             for (int k = 0; k < KL; ++k) R.pre[k] = bload<T>(rs_in, lofs[k], so);
         }
@@ -200,8 +206,8 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
     const int c0 = t.z0 - 1;  // even: chunks start at odd z0
     int nsteps = t.z1 - t.z0 + 3;
     nsteps += nsteps & 1;     // a last chunk of odd depth gets one idle step
-    const int rem = nsteps % SW_UNROLL;  // even
-    const int cm = c0 + rem;  // first step of the unrolled loop
+    const int pad = (SW_UNROLL - nsteps % SW_UNROLL) % SW_UNROLL;  // even
+    const int cm = c0 - pad;  // first step
     // plane z lives in slot (z - cm + 2) mod 5
     auto slot = [&](int z) { return ((z - cm + 2) % NS + NS) % NS; };
 
@@ -216,25 +222,33 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
         }
     }
 
-    Regs R0, R1;
+    constexpr int NR = ZERO ? 5 : 2;  // divides the unroll
+    Regs RS[NR];
     if (ZERO) {
 #pragma unroll
-        for (int k = 0; k < KL; ++k) R0.pre[k] = R1.pre[k] = T(0);
+        for (int i = 0; i < NR; ++i)
+#pragma unroll
+            for (int k = 0; k < KL; ++k) RS[i].pre[k] = T(0);
     }
-    load_regs(R0, c0, Step<0>{});
-    load_regs(R1, c0 + 1, Step<1>{});
+    load_regs(RS[0], cm, Step<0>{});
+    load_regs(RS[1], cm + 1, Step<1>{});
+    if constexpr (NR == 5) {
+        load_regs(RS[2], cm + 2, Step<0>{});
+        load_regs(RS[3], cm + 3, Step<1>{});
+        load_regs(RS[4], cm + 4, Step<0>{});
+    }
 
     // one step on plane c with parity PR; s_*: slots of planes c-2 .. c+1
-    auto step = [&](int c, auto parc, int s_m2, int s_m1, int s_c, int s_p1) {
+    auto step = [&](int c, auto parc, auto setc, int s_m2, int s_m1, int s_c, int s_p1) {
         constexpr int PR = decltype(parc)::value;
-        Regs &R = PR ? R1 : R0;
+        Regs &R = RS[decltype(setc)::value];
         {
             T *P = sp + s_p1 * PS;  // beyond zhi the slot holds a plane nobody reads
 #pragma unroll
             for (int k = 0; k < KL; ++k) P[tid + k * NT] = R.pre[k];
         }
         const T rr_cur = R.rr, rb_cur = R.rb;
-        load_regs(R, c + 2, parc);
+        load_regs(R, c + NR, Step<(PR + NR) & 1>{});
         __syncthreads();
         if (c >= 1 && c <= g.n2 && rok[PR]) {
             T *P = sp + s_c * PS;
@@ -271,12 +285,10 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
         }
     };
 
-    for (int c = c0; c < cm; c += 2) {
-        step(c, Step<0>{}, slot(c - 2), slot(c - 1), slot(c), slot(c + 1));
-        step(c + 1, Step<1>{}, slot(c - 1), slot(c), slot(c + 1), slot(c + 2));
-    }
-    // unrolled: step J on plane c = cm + 10m + J keeps plane c in slot (J + 2) mod 5
-#define MF3D_RBS_STEP(J) step(c + J, Step<(J) & 1>{}, (J) % NS, ((J) + 1) % NS, ((J) + 2) % NS, ((J) + 3) % NS)
+    // step J on plane c = cm + 10m + J keeps plane c in slot (J + 2) mod 5
+    // and uses register set J mod NR
+#define MF3D_RBS_STEP(J)                                                                                  \
+    step(c + J, Step<(J) & 1>{}, Step<(J) % NR>{}, (J) % NS, ((J) + 1) % NS, ((J) + 2) % NS, ((J) + 3) % NS)
     for (int c = cm; c < c0 + nsteps; c += SW_UNROLL) {
         MF3D_RBS_STEP(0);
         MF3D_RBS_STEP(1);
