@@ -1171,8 +1171,11 @@ int build(mf3d_mg *mg) {
 // others with 4-row tiles) and keep the fastest.  Each candidate is launched
 // once to warm up and then timed over 3 launches with HIP events on the
 // handle's stream, on the level's own (zeroed) arrays; the fields are zeroed
-// again afterwards.  The tiling never changes a result.  MF3D_AUTOTUNE=0
-// keeps the default depths.
+// again afterwards.  Of the depths within 3 % of the fastest the deepest is
+// kept (fewer halo planes read).  The tiling never changes a result.
+// MF3D_AUTOTUNE=0 keeps the default depths; MF3D_SWEEP_TZ=<even depth>
+// uses that depth on every level without timing (to pin the tiling of a
+// profile).
 // Chosen depths are remembered per process, keyed by precision, coefficient
 // form, tile rows and the level's block extents, so later handles of the
 // same shape (the serving pool's) skip the timing.
@@ -1183,6 +1186,14 @@ template <typename T, bool VAR>
 int autotune_sweeps(mf3d_mg *mg) {
     const char *env = std::getenv("MF3D_AUTOTUNE");
     if (env && env[0] == '0') return 0;
+    int pin_tz = 0;
+    if (const char *p = std::getenv("MF3D_SWEEP_TZ")) {
+        pin_tz = std::atoi(p);
+        if (pin_tz < 2 || (pin_tz & 1)) {
+            mg->err = "MF3D_SWEEP_TZ must be an even depth of at least 2";
+            return MF3D_EINVAL;
+        }
+    }
     auto key_of = [&](const Level &V) {
         std::string k = std::to_string(sizeof(T)) + (VAR ? "v" : "c") + std::to_string(V.sw_ty);
         for (const BlockLevel &g : V.hb)
@@ -1202,6 +1213,7 @@ int autotune_sweeps(mf3d_mg *mg) {
             auto it = g_tune_tz.find(key);
             if (it != g_tune_tz.end()) cached = it->second;
         }
+        if (pin_tz) cached = pin_tz;
         if (cached) {
             auto t = xcd_order(sweep_tiles(V.hb, V.sw_ty, cached));
             Tile *dst = nullptr;
@@ -1212,9 +1224,7 @@ int autotune_sweeps(mf3d_mg *mg) {
         }
         int nzmax = 1;
         for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
-        float best = 1e30f;
-        int best_tz = 0;
-        std::vector<Tile> best_t;
+        std::vector<std::pair<int, float>> timed;  // (depth, ms)
         Tile *tmp = nullptr;
         // powers of two, and depths whose step count (tz + 2) is a whole
         // number of the sweep's 10-step unrolled blocks
@@ -1240,13 +1250,15 @@ int autotune_sweeps(mf3d_mg *mg) {
             V.t_sw = keep;
             V.n_sw = keep_n;
             (void)hipFree(tmp);
-            if (ms < best) {
-                best = ms;
-                best_tz = tz;
-                best_t = std::move(t);
-            }
+            timed.emplace_back(tz, ms);
         }
-        if (!best_t.empty()) {
+        float best = 1e30f;
+        for (const auto &e : timed) best = std::min(best, e.second);
+        int best_tz = 0;
+        for (const auto &e : timed)
+            if (e.second <= 1.03f * best && best < 1e29f) best_tz = std::max(best_tz, e.first);
+        if (best_tz) {
+            const auto best_t = xcd_order(sweep_tiles(V.hb, V.sw_ty, best_tz));
             {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
                 g_tune_tz[key] = best_tz;
