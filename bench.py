@@ -41,6 +41,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rbs_256cubed_L7_traffic.md")
 VEL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_velmas_256cubed_L7.md")
 
 
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), or None."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(form, path=TRAFFIC_FILE, key="traffic_bytes_per_launch:"):
     """HBM bytes per launch of the 256^3, 7-level workload from a committed
     rocprofv3 PMC summary: FETCH_SIZE x 2 + WRITE_SIZE, the factors fixed by
@@ -400,7 +412,22 @@ def main():
                "sample": f"one full FP64 solve of the same {n}^3 problem ({k_o} cycles, "
                          f"{t_cpu:.1f} s) by the C restatement of the reference (oracle/mg_oracle.c), "
                          f"single thread",
-               "fp64_gpu_bitwise_equal": fp64_bitwise}
+               "fp64_gpu_bitwise_equal": fp64_bitwise,
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+        # the same solve on every core the process may use (the OpenMP build
+        # of the same source; its results are bitwise the serial build's)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+        os.environ["OMP_NUM_THREADS"] = str(threads)  # read when libgomp loads
+        o = OracleMG(grid.blocks, L, omp=True)
+        o.coeff()
+        t0 = time.perf_counter()
+        p_m, _, _, k_m = o.solve(f_host, args.tol, args.maxcy)
+        t_mc = time.perf_counter() - t0
+        o.close()
+        cpu["multicore"] = {"value": round(1.0 / t_mc, 5), "unit": "solves/s", "cores": threads,
+                            "sample": f"the same solve ({k_m} cycles, {t_mc:.2f} s) by the OpenMP build "
+                                      f"of oracle/mg_oracle.c (plane loops parallel, reductions serial)",
+                            "bitwise_equal_to_single_thread": bool(np.array_equal(p_m[0], p_o[0]))}
 
     velocity = None
     if not args.no_velocity:

@@ -21,6 +21,13 @@
  * FMA contraction), so results are bitwise comparable with the compiled
  * reference.  Build with -ffp-contract=off.
  *
+ * Built with -fopenmp (liborc_mg_omp.so, the multi-core CPU baseline) the
+ * red-black colour passes, residuals, restriction, prolongation and the
+ * outer update run their plane loops in parallel.  Each of those loops
+ * writes disjoint cells from values no other iteration writes, and every
+ * reduction stays serial, so the results are bitwise those of the serial
+ * build.
+ *
  * Scope notes (what is not restated):
  *  - refined-block (rdv unequal) exchange branches and the injection branch
  *    for blocks with fewer levels (mgsolver.for:719-732, 807-819);
@@ -412,6 +419,7 @@ static void rb_colour(orc_mg *mg, int b, int l, int parity) {
     const double *c = mg->cof[SLOT(mg, b, l)];
     const double *rh = mg->rhs[SLOT(mg, b, l)];
     size_t sy = (size_t)(n[0] + 2), sz = sy * (size_t)(n[1] + 2);
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= n[2]; ++k)
         for (int j = 1; j <= n[1]; ++j)
             for (int i = 1; i <= n[0]; ++i) {
@@ -566,6 +574,7 @@ static void residual_block(orc_mg *mg, int b, int l, const double *p, const doub
     size_t N = isz(n);
     const double *c = mg->cof[SLOT(mg, b, l)];
     size_t sy = (size_t)(n[0] + 2), sz = sy * (size_t)(n[1] + 2);
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= n[2]; ++k)
         for (int j = 1; j <= n[1]; ++j)
             for (int i = 1; i <= n[0]; ++i) {
@@ -599,6 +608,7 @@ void orc_restrict(orc_mg *mg, int lc) {
         memset(rc, 0, isz(nc) * sizeof(double)); /* :659-667 */
         double *resf = malloc(isz(nf) * sizeof(double));
         residual_block(mg, b, lf, mg->phi[SLOT(mg, b, lf)], mg->rhs[SLOT(mg, b, lf)], resf);
+#pragma omp parallel for schedule(static)
         for (int k = 1; k <= nc[2]; ++k)
             for (int j = 1; j <= nc[1]; ++j)
                 for (int i = 1; i <= nc[0]; ++i) {
@@ -629,6 +639,8 @@ void orc_correct(orc_mg *mg, int lf) {
         const int32_t *nc = DIMS(mg, b, lc), *nf = DIMS(mg, b, lf);
         const double *pc = mg->phi[SLOT(mg, b, lc)];
         double *pfi = mg->phi[SLOT(mg, b, lf)];
+        /* coarse plane kc writes fine planes 2kc and 2kc+1 only */
+#pragma omp parallel for schedule(static)
         for (int kc = 0; kc <= nc[2]; ++kc)
             for (int ic = 0; ic <= nc[0]; ++ic)
                 for (int jc = 0; jc <= nc[1]; ++jc) {
@@ -753,6 +765,7 @@ int orc_solve(orc_mg *mg, const double *f, double *p, double tol, int maxcy, int
         size_t og = 0;
         for (int b = 0; b < mg->nb; ++b) { /* newsolv_mg.for:38-48, ppref == 0 */
             const int32_t *n = DIMS(mg, b, 0);
+#pragma omp parallel for schedule(static)
             for (int kk = 1; kk <= n[2]; ++kk)
                 for (int j = 1; j <= n[1]; ++j)
                     for (int i = 1; i <= n[0]; ++i) {

@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc_mg.so")
+LIB_OMP_PATH = os.path.join(HERE, "liborc_mg_omp.so")  # same code, OpenMP plane loops
 REF_BIN = os.path.join(HERE, "_ref", "mg_ref")
 REF_VEL_BIN = os.path.join(HERE, "_ref", "mg_ref_vel")
 
@@ -27,15 +28,17 @@ class OrcBlock(ctypes.Structure):
                 ("bc", ctypes.c_int32 * 6)]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(omp: bool = False):
+    """The serial oracle, or (omp) its OpenMP build, whose results are
+    bitwise the serial build's; threads follow OMP_NUM_THREADS."""
+    path = LIB_OMP_PATH if omp else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
         vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
         L.orc_create.restype = vp
         L.orc_create.argtypes = [ip, ctypes.POINTER(OrcBlock), ip, ip, ip, ip]
@@ -60,8 +63,8 @@ def lib():
         for name in ("orc_restrict", "orc_correct", "orc_exchange", "orc_bound"):
             getattr(L, name).argtypes = [vp, ip]
         L.orc_residual.argtypes = [vp, ip, dp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dptr(a: np.ndarray):
@@ -77,11 +80,12 @@ class OracleMG:
     """FP64 multigrid of the reference, restated on the CPU."""
 
     def __init__(self, blocks, levels: int, lsm: bool = False, nu_pre: int = 2, nu_post: int = 2,
-                 coarse_op: str = "reference", smoother: int = 3):
+                 coarse_op: str = "reference", smoother: int = 3, omp: bool = False):
         """lsm: variable (density) coefficients; coarse_op "face" builds their
         coarse operators by face-coefficient restriction instead of the
         reference's density hierarchy; smoother: mg_itrsch (1 lexicographic
-        GS, 2 ADI-TDMA, 3 red-black GS)."""
+        GS, 2 ADI-TDMA, 3 red-black GS); omp: run on the OpenMP build."""
+        self._L = lib(omp)
         self.blocks = list(blocks)
         self.levels = levels
         self.lsm = lsm
@@ -93,30 +97,30 @@ class OracleMG:
             arr[i].per[:] = list(b.per)
             arr[i].bc[:] = list(b.bc)
         mode = (2 if coarse_op == "face" else 1) if lsm else 0
-        self._h = lib().orc_create(len(self.blocks), arr, levels, mode, nu_pre, nu_post)
+        self._h = self._L.orc_create(len(self.blocks), arr, levels, mode, nu_pre, nu_post)
         if not self._h:
             raise ValueError("oracle rejected the block set (extent not divisible by 2**(L-1)?)")
-        if lib().orc_set_smoother(self._h, smoother) != 0:
+        if self._L.orc_set_smoother(self._h, smoother) != 0:
             raise ValueError(f"unknown smoother {smoother}")
 
     def close(self):
         if self._h:
-            lib().orc_destroy(self._h)
+            self._L.orc_destroy(self._h)
             self._h = None
 
     __del__ = close
 
     def dims(self, block: int, level: int):
         out = (ctypes.c_int32 * 3)()
-        lib().orc_dims(self._h, block, level, out)
+        self._L.orc_dims(self._h, block, level, out)
         return tuple(out)
 
     def set_density(self, dens_blocks):
         d = _concat(dens_blocks)
-        lib().orc_set_density(self._h, _dptr(d))
+        self._L.orc_set_density(self._h, _dptr(d))
 
     def coeff(self):
-        lib().orc_coeff(self._h)
+        self._L.orc_coeff(self._h)
 
     def vcycle(self, rhs_blocks, pp_blocks=None):
         """One mgkcyc; returns pp per block with its ghost layer, shaped (z+2, y+2, x+2)."""
@@ -126,7 +130,7 @@ class OracleMG:
             pp = np.zeros(sum(int(np.prod(s)) for s in shapes))
         else:
             pp = _concat(pp_blocks)
-        lib().orc_vcycle(self._h, _dptr(rhs), _dptr(pp))
+        self._L.orc_vcycle(self._h, _dptr(rhs), _dptr(pp))
         out, o = [], 0
         for s in shapes:
             m = int(np.prod(s))
@@ -141,7 +145,7 @@ class OracleMG:
         p = np.zeros_like(f)
         rho = np.zeros(maxcy + 1)
         rmax = np.zeros(maxcy + 1)
-        fn = lib().orc_solve_gcr if gcr else lib().orc_solve
+        fn = self._L.orc_solve_gcr if gcr else self._L.orc_solve
         k = fn(self._h, _dptr(f), _dptr(p), tol, maxcy, min_cycles, _dptr(rho), _dptr(rmax))
         return self._split(p), rho[:k + 1].copy(), rmax[:k + 1].copy(), k
 
@@ -157,43 +161,43 @@ class OracleMG:
     # -- unit access (views into oracle memory; copy before the next call) --
     def phi(self, block: int, level: int) -> np.ndarray:
         nx, ny, nz = self.dims(block, level)
-        p = lib().orc_phi(self._h, block, level)
+        p = self._L.orc_phi(self._h, block, level)
         return np.ctypeslib.as_array(p, shape=((nz + 2) * (ny + 2) * (nx + 2),)).reshape(nz + 2, ny + 2, nx + 2)
 
     def rhs(self, block: int, level: int) -> np.ndarray:
         nx, ny, nz = self.dims(block, level)
-        p = lib().orc_rhs(self._h, block, level)
+        p = self._L.orc_rhs(self._h, block, level)
         return np.ctypeslib.as_array(p, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
 
     def cof(self, block: int, level: int, which: int) -> np.ndarray:
         nx, ny, nz = self.dims(block, level)
-        p = lib().orc_cof(self._h, block, level, which)
+        p = self._L.orc_cof(self._h, block, level, which)
         return np.ctypeslib.as_array(p, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
 
     def dens(self, block: int, level: int) -> np.ndarray:
         nx, ny, nz = self.dims(block, level)
-        p = lib().orc_dens(self._h, block, level)
+        p = self._L.orc_dens(self._h, block, level)
         return np.ctypeslib.as_array(p, shape=(nz * ny * nx,)).reshape(nz, ny, nx)
 
     def relax(self, level: int, sweeps: int):
-        lib().orc_relax(self._h, level, sweeps)
+        self._L.orc_relax(self._h, level, sweeps)
 
     def restrict(self, coarse_level: int):
-        lib().orc_restrict(self._h, coarse_level)
+        self._L.orc_restrict(self._h, coarse_level)
 
     def correct(self, fine_level: int):
-        lib().orc_correct(self._h, fine_level)
+        self._L.orc_correct(self._h, fine_level)
 
     def exchange(self, level: int):
-        lib().orc_exchange(self._h, level)
+        self._L.orc_exchange(self._h, level)
 
     def bound(self, level: int):
-        lib().orc_bound(self._h, level)
+        self._L.orc_bound(self._h, level)
 
     def residual(self, level: int):
         tot = sum(int(np.prod(self.dims(b, level))) for b in range(len(self.blocks)))
         r = np.zeros(tot)
-        lib().orc_residual(self._h, level, _dptr(r))
+        self._L.orc_residual(self._h, level, _dptr(r))
         out, o = [], 0
         for b in range(len(self.blocks)):
             nx, ny, nz = self.dims(b, level)
