@@ -59,7 +59,7 @@ struct Level {
     bool adi_rhs = false;
     int adi_cells = 0, adi_sstride = 0;
     void *adi_apr[3] = {}, *adi_a[3] = {};  // the LDS kernel's Thomas coefficients per line axis
-    int sw_ty = 4;  // sweep tile rows
+    int sw_ty = 4;  // sweep tile rows of the march; 0: 3-D tiles (k_rb3d)
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
     bool own_ghosts = false;  // lone block: the sweep writes the ghosts, no gather
@@ -361,6 +361,19 @@ std::vector<Tile> sweep_tiles(const std::vector<BlockLevel> &hb, int ty, int tz 
     return t;
 }
 
+// 3-D sweep tiles (k_rb3d): R3_TX x R3_TY x R3_TZ cells, clipped to the block.
+std::vector<Tile> tiles_3d(const std::vector<BlockLevel> &hb) {
+    std::vector<Tile> t;
+    for (int b = 0; b < (int)hb.size(); ++b) {
+        const BlockLevel &g = hb[b];
+        for (int z = 1; z <= g.n2; z += R3_TZ)
+            for (int y = 1; y <= g.n1; y += R3_TY)
+                for (int x = 1; x <= g.n0; x += R3_TX)
+                    t.push_back(Tile{b, x, y, z, std::min(g.n2, z + R3_TZ - 1)});
+    }
+    return t;
+}
+
 // The dispatcher hands workgroup i to XCD i mod 8.  Reorder a spatially
 // ordered tile list so that XCD k receives the k-th contiguous eighth:
 // tiles that share halo planes then run under the same L2.  Results do not
@@ -479,6 +492,12 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
     T *pout = (T *)L.phi[1 - L.cur];
     const uint32_t bytes = (uint32_t)(L.cells_gh * (long long)sizeof(T));
     const int own = L.own_ghosts ? 1 : 0;
+    if (L.sw_ty == 0) {  // 3-D tiles
+        hipLaunchKernelGGL((k_rb3d<T, VAR, ZERO, RCP>), dim3(L.n_sw), dim3(R3_NT), 0, mg->stream, pin, pout,
+                           (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
+                           L.t_sw, done);
+        return;
+    }
     if (L.sw_ty == 8)
         hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
@@ -1170,12 +1189,14 @@ int build(mf3d_mg *mg) {
 // over chunk depths tz from 2 to 64 (the finest level with 8-row tiles, the
 // others with 4-row tiles) and keep the fastest.  Each candidate is launched
 // once to warm up and then timed over 3 launches with HIP events on the
-// handle's stream, on the level's own (zeroed) arrays; the fields are zeroed
-// again afterwards.  Of the depths within 3 % of the fastest the deepest is
-// kept (fewer halo planes read).  The tiling never changes a result.
-// MF3D_AUTOTUNE=0 keeps the default depths; MF3D_SWEEP_TZ=<even depth>
-// uses that depth on every level without timing (to pin the tiling of a
-// profile).
+// handle's stream (12 timed launches on levels under 2^20 cells), on the
+// level's own (zeroed) arrays; the fields are zeroed again afterwards.  Of
+// the depths within 3 % of the fastest the deepest is kept (fewer halo
+// planes read).  The one-pass 3-D tile kernel (k_rb3d) is timed too and
+// taken where it beats the best march.  The tiling never changes a result.  MF3D_AUTOTUNE=0 keeps the default depths;
+// MF3D_SWEEP_TZ=<even depth> uses that march depth on every level without
+// timing (to pin the tiling of a profile); MF3D_SWEEP_3D=1 uses the 3-D
+// tiles on every level, =0 never.
 // Chosen depths are remembered per process, keyed by precision, coefficient
 // form, tile rows and the level's block extents, so later handles of the
 // same shape (the serving pool's) skip the timing.
@@ -1194,6 +1215,8 @@ int autotune_sweeps(mf3d_mg *mg) {
             return MF3D_EINVAL;
         }
     }
+    int force3d = -1;  // MF3D_SWEEP_3D: 1 every level, 0 never, unset timed
+    if (const char *p = std::getenv("MF3D_SWEEP_3D")) force3d = p[0] == '1' ? 1 : 0;
     auto key_of = [&](const Level &V) {
         std::string k = std::to_string(sizeof(T)) + (VAR ? "v" : "c") + std::to_string(V.sw_ty);
         for (const BlockLevel &g : V.hb)
@@ -1203,73 +1226,82 @@ int autotune_sweeps(mf3d_mg *mg) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    // tiles of a choice: a march depth, or kTz3d for the 3-D tiles
+    constexpr int kTz3d = -1;
+    auto tiles_of = [&](const Level &V, int ty, int tz) {
+        return xcd_order(tz == kTz3d ? tiles_3d(V.hb) : sweep_tiles(V.hb, ty, tz));
+    };
+    // 1 warm-up + 3 timed sweeps over a candidate tiling (12 on levels under
+    // 2^20 cells, whose few-microsecond sweeps time noisily); ms, 1e30 on failure
+    auto time_tiles = [&](Level &V, const std::vector<Tile> &t, int ty) -> float {
+        const int reps = V.cells_gh < (1LL << 20) ? 12 : 3;
+        Tile *tmp = nullptr;
+        if (hipMalloc((void **)&tmp, sizeof(Tile) * t.size()) != hipSuccess) return 1e30f;
+        Tile *keep = V.t_sw;
+        const int keep_n = V.n_sw, keep_ty = V.sw_ty;
+        V.t_sw = tmp;
+        V.n_sw = (int)t.size();
+        V.sw_ty = ty;
+        float ms = 1e30f;
+        if (hipMemcpyAsync(tmp, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, mg->stream) ==
+            hipSuccess) {
+            sweep_launch<T, VAR, false>(mg, V, nullptr);
+            (void)hipEventRecord(e0, mg->stream);
+            for (int r = 0; r < reps; ++r) sweep_launch<T, VAR, false>(mg, V, nullptr);
+            (void)hipEventRecord(e1, mg->stream);
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                ms = 1e30f;
+        }
+        V.t_sw = keep;
+        V.n_sw = keep_n;
+        V.sw_ty = keep_ty;
+        (void)hipFree(tmp);
+        return ms;
+    };
     int rc = 0;
     for (int l = 0; l < std::min(mg->lt, (int)mg->lv.size()) && !rc; ++l) {
         Level &V = mg->lv[l];
+        const int ty = V.sw_ty;  // the march's tile rows on this level
         const std::string key = key_of(V);
-        int cached = 0;
+        int choice = 0;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune_tz.find(key);
-            if (it != g_tune_tz.end()) cached = it->second;
+            if (it != g_tune_tz.end()) choice = it->second;
         }
-        if (pin_tz) cached = pin_tz;
-        if (cached) {
-            auto t = xcd_order(sweep_tiles(V.hb, V.sw_ty, cached));
-            Tile *dst = nullptr;
-            if ((rc = upload(mg, &dst, t))) break;
-            V.t_sw = dst;
-            V.n_sw = (int)t.size();
-            continue;
-        }
-        int nzmax = 1;
-        for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
-        std::vector<std::pair<int, float>> timed;  // (depth, ms)
-        Tile *tmp = nullptr;
-        // powers of two, and depths whose step count (tz + 2) is a whole
-        // number of the sweep's 10-step unrolled blocks
-        static const int kTz[] = {2, 4, 8, 16, 18, 28, 32, 38, 48, 58, 64};
-        for (int tz : kTz) {
-            if (tz > std::max(2, nzmax)) break;
-            auto t = xcd_order(sweep_tiles(V.hb, V.sw_ty, tz));
-            if (hipMalloc((void **)&tmp, sizeof(Tile) * t.size()) != hipSuccess) break;
-            Tile *keep = V.t_sw;
-            const int keep_n = V.n_sw;
-            V.t_sw = tmp;
-            V.n_sw = (int)t.size();
-            float ms = 0.f;
-            if (hipMemcpyAsync(tmp, t.data(), sizeof(Tile) * t.size(), hipMemcpyHostToDevice, mg->stream) ==
-                hipSuccess) {
-                sweep_launch<T, VAR, false>(mg, V, nullptr);
-                (void)hipEventRecord(e0, mg->stream);
-                for (int r = 0; r < 3; ++r) sweep_launch<T, VAR, false>(mg, V, nullptr);
-                (void)hipEventRecord(e1, mg->stream);
-                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-                    ms = 1e30f;
+        if (force3d == 1) choice = kTz3d;
+        else if (pin_tz) choice = pin_tz;
+        else if (force3d == 0 && choice == kTz3d) choice = 0;
+        if (!choice) {
+            int nzmax = 1;
+            for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
+            // powers of two, and depths whose step count (tz + 2) is a whole
+            // number of the march's 10-step unrolled blocks
+            static const int kTz[] = {2, 4, 8, 16, 18, 28, 32, 38, 48, 58, 64};
+            std::vector<std::pair<int, float>> timed;  // (depth, ms)
+            for (int tz : kTz) {
+                if (tz > std::max(2, nzmax)) break;
+                timed.emplace_back(tz, time_tiles(V, tiles_of(V, ty, tz), ty));
             }
-            V.t_sw = keep;
-            V.n_sw = keep_n;
-            (void)hipFree(tmp);
-            timed.emplace_back(tz, ms);
-        }
-        float best = 1e30f;
-        for (const auto &e : timed) best = std::min(best, e.second);
-        int best_tz = 0;
-        for (const auto &e : timed)
-            if (e.second <= 1.03f * best && best < 1e29f) best_tz = std::max(best_tz, e.first);
-        if (best_tz) {
-            const auto best_t = xcd_order(sweep_tiles(V.hb, V.sw_ty, best_tz));
-            {
+            float best = 1e30f;
+            for (const auto &e : timed) best = std::min(best, e.second);
+            for (const auto &e : timed)
+                if (e.second <= 1.03f * best && best < 1e29f) choice = std::max(choice, e.first);
+            if (force3d != 0 && time_tiles(V, tiles_of(V, ty, kTz3d), 0) < best) choice = kTz3d;
+            if (choice) {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
-                g_tune_tz[key] = best_tz;
+                g_tune_tz[key] = choice;
             }
-            Tile *dst = nullptr;
-            if ((rc = upload(mg, &dst, best_t))) break;
-            V.t_sw = dst;
-            V.n_sw = (int)best_t.size();
+            fill<T>(mg, (T *)V.phi[0], V.cells_gh);
+            fill<T>(mg, (T *)V.phi[1], V.cells_gh);
         }
-        fill<T>(mg, (T *)V.phi[0], V.cells_gh);
-        fill<T>(mg, (T *)V.phi[1], V.cells_gh);
+        if (!choice) continue;
+        const auto t = tiles_of(V, ty, choice);
+        Tile *dst = nullptr;
+        if ((rc = upload(mg, &dst, t))) break;
+        V.t_sw = dst;
+        V.n_sw = (int)t.size();
+        V.sw_ty = choice == kTz3d ? 0 : ty;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -304,6 +304,140 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
 #undef MF3D_RBS_STEP
 }
 
+// The same red-black sweep (mgsolver.for:549-601) over 3-D tiles, without
+// a march, for levels where the march's per-step latency dominates (few
+// columns, few planes).  A workgroup loads its R3_TX x R3_TY x R3_TZ tile of
+// pin with a 2-cell halo into LDS, updates the first colour on the tile plus
+// a 1-cell ring in place, then computes the second colour on the tile, and
+// writes every interior cell of the tile (with its ghost images on a lone
+// block, as k_rbs does).  First-colour cells in the ring are recomputed by
+// the neighbouring tiles from the same inputs, so every stored value is the
+// one k_rbs stores: same update expression, same operand order.
+constexpr int R3_TX = 32, R3_TY = 8, R3_TZ = 8, R3_NT = 1024;
+
+template <typename T, bool VAR, bool ZERO, bool RCP>
+__global__ __launch_bounds__(R3_NT) void k_rb3d(const T *__restrict__ pin, T *__restrict__ pout,
+                                                const T *__restrict__ rhs, const T *__restrict__ cx,
+                                                const T *__restrict__ cy, const T *__restrict__ cz, uint32_t bytes,
+                                                int own_ghosts, const BlockLevel *__restrict__ bls,
+                                                const Tile *__restrict__ tiles, const int *__restrict__ done) {
+    if (done && *done) return;
+    constexpr int LX = R3_TX + 4, LY = R3_TY + 4, LZ = R3_TZ + 4, LXY = LX * LY, LN = LXY * LZ;
+    constexpr int KL = (LN + R3_NT - 1) / R3_NT;
+    constexpr int RX = (R3_TX + 2) / 2;                       // first-colour cells per ring row
+    constexpr int NRED = RX * (R3_TY + 2) * (R3_TZ + 2);      // first-colour cells of tile + ring
+    constexpr int KR = (NRED + R3_NT - 1) / R3_NT;
+    constexpr int NPAIR = (R3_TX / 2) * R3_TY * R3_TZ;        // (second, first) colour pairs of the tile
+    constexpr int KP = NPAIR / R3_NT;
+    static_assert(NPAIR % R3_NT == 0, "whole pairs per thread");
+    constexpr uint32_t W = sizeof(T);
+    __shared__ T sp[LN];
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b];
+    const auto rs_in = make_rsrc(pin, bytes), rs_rhs = make_rsrc(rhs, bytes), rs_out = make_rsrc(pout, bytes);
+    const int tid = threadIdx.x;
+    const int xb = t.x0 - 2, yb = t.y0 - 2, zb = t.z0 - 2;
+    const int sy = (int)g.sy;
+    const uint32_t pb = (uint32_t)g.sz * W, ob = (uint32_t)g.off * W;
+    // cell (x, y, z) of the ghosted block: in-bounds byte offset of its row
+    auto roff = [&](int x, int y) { return ob + (uint32_t)(x + sy * y) * W; };
+
+    // the tile with its halo (positions outside the ghosted block feed no
+    // update and stay zero)
+    {
+        T v[KL];
+#pragma unroll
+        for (int k = 0; k < KL; ++k) {
+            const int p = tid + k * R3_NT;
+            const int lz = p / LXY, r = p - lz * LXY, ly = r / LX, lx = r - ly * LX;
+            const int x = xb + lx, y = yb + ly, z = zb + lz;
+            const bool ok = p < LN && x >= 0 && x <= g.n0 + 1 && y >= 0 && y <= g.n1 + 1 && z >= 0 && z <= g.n2 + 1;
+            v[k] = T(0);
+            if (!ZERO && ok) v[k] = bload<T>(rs_in, roff(x, y), (uint32_t)z * pb);
+        }
+#pragma unroll
+        for (int k = 0; k < KL; ++k)
+            if (tid + k * R3_NT < LN) sp[tid + k * R3_NT] = v[k];
+    }
+    // one update from LDS, as k_rbs's (coefficients from coefs)
+    auto update = [&](int l, int x, int y, int z, T r) -> T {
+        T aw, ae, as, an, ab, at, ap;
+        const long long q = g.off + x + (long long)sy * y + g.sz * z;
+        coefs<T, VAR>(g, cx, cy, cz, q, x, y, z, aw, ae, as, an, ab, at, ap);
+        T s = aw * sp[l - 1];
+        s = s + ae * sp[l + 1];
+        s = s + as * sp[l - LX];
+        s = s + an * sp[l + LX];
+        s = s + ab * sp[l - LXY];
+        s = s + at * sp[l + LXY];
+        return RCP ? (r - s) * (T(1) / ap) : (r - s) / ap;
+    };
+    // first colour ((x+y+z) odd) on the tile and its ring, interior cells only
+    int rx[KR], ry[KR], rz[KR];
+    bool rv[KR];
+    T rr[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int m = tid + k * R3_NT;
+        const int row = m / RX, i = m - row * RX;
+        const int z = t.z0 - 1 + row / (R3_TY + 2), y = t.y0 - 1 + row % (R3_TY + 2);
+        const int xs = t.x0 - 1;
+        const int x = xs + ((1 - y - z - xs) & 1) + 2 * i;
+        rx[k] = x; ry[k] = y; rz[k] = z;
+        rv[k] = m < NRED && x >= 1 && x <= g.n0 && y >= 1 && y <= g.n1 && z >= 1 && z <= g.n2;
+        rr[k] = rv[k] ? bload<T>(rs_rhs, roff(x, y), (uint32_t)z * pb) : T(0);
+    }
+    // second-colour cells of this thread's pairs and their RHS
+    int px[KP], py[KP], pz[KP];
+    T pr[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const int m = tid + k * R3_NT;
+        const int row = m / (R3_TX / 2), i = m - row * (R3_TX / 2);
+        const int z = t.z0 + row / R3_TY, y = t.y0 + row % R3_TY;
+        const int x = t.x0 + 2 * i + ((y + z + t.x0) & 1);  // (x+y+z) even
+        px[k] = x; py[k] = y; pz[k] = z;
+        const bool v = x <= g.n0 && y <= g.n1 && z <= g.n2;
+        pr[k] = v ? bload<T>(rs_rhs, roff(x, y), (uint32_t)z * pb) : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KR; ++k)
+        if (rv[k]) {
+            const int l = (rx[k] - xb) + LX * ((ry[k] - yb) + LY * (rz[k] - zb));
+            sp[l] = update(l, rx[k], ry[k], rz[k], rr[k]);
+        }
+    __syncthreads();
+    auto store = [&](T v, int x, int y, int z) {
+        bstore<T>(rs_out, v, roff(x, y), (uint32_t)z * pb);
+        if (!own_ghosts) return;
+        const int ix = ghost_image(x, g.n0, g.neu & kNeuW, g.neu & kNeuE);
+        const int iy = ghost_image(y, g.n1, g.neu & kNeuS, g.neu & kNeuN);
+        const int iz = ghost_image(z, g.n2, g.neu & kNeuB, g.neu & kNeuT);
+        if (ix < 0 && iy < 0 && iz < 0) return;
+        auto img = [&](int xx, int yy, int zz) { bstore<T>(rs_out, v, roff(xx, yy), (uint32_t)zz * pb); };
+        if (ix >= 0) img(ix, y, z);
+        if (iy >= 0) img(x, iy, z);
+        if (ix >= 0 && iy >= 0) img(ix, iy, z);
+        if (iz >= 0) {
+            img(x, y, iz);
+            if (ix >= 0) img(ix, y, iz);
+            if (iy >= 0) img(x, iy, iz);
+            if (ix >= 0 && iy >= 0) img(ix, iy, iz);
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const int x = px[k], y = py[k], z = pz[k];
+        if (y > g.n1 || z > g.n2) continue;
+        // the first-colour partner: the other cell of the pair (x0+2i, x0+2i+1)
+        const int xr = ((x - t.x0) & 1) ? x - 1 : x + 1;
+        const int l = (x - xb) + LX * ((y - yb) + LY * (z - zb));
+        if (x <= g.n0) store(update(l, x, y, z, pr[k]), x, y, z);
+        if (xr <= g.n0) store(sp[l + (xr - x)], xr, y, z);
+    }
+}
+
 // Ghost fill from a region table: exchangepp + mgbound + exchbc_mgpp of the
 // reference in one gather (every ghost takes the value of the interior cell
 // it stands for).  Sources are interior cells, which this kernel never
