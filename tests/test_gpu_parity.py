@@ -97,3 +97,32 @@ def test_deterministic_repeat():
     assert np.array_equal(r1.hist, r2.hist)
     for a, b in zip(p1, p2):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["single16_L4", "multi_uneven_L3", "lsm_multi_L3", "periodic_multi_L3",
+                                  "mixed_bc_multi_L3"])
+def test_both_sweep_kernels_bitwise(name, monkeypatch):
+    """The one-pass 3-D tile sweep (k_rb3d) and the z-march (k_rbs), each
+    forced on every level through MF3D_SWEEP_3D (the tuner otherwise picks
+    per level by timing): in FP64 both give the oracle's pressure bitwise;
+    in mixed precision they give each other's bitwise (same FP32 update
+    expression, same operand order)."""
+    grid, L, lsm = case(name)
+    f, dens = inputs(grid, lsm, seed=11)
+    ncyc = 3
+    p_o, _, _, _ = _oracle(grid, L, lsm, dens).solve(f, 0.0, ncyc)
+    got = {}
+    for tiles in ("3d", "march"):
+        monkeypatch.setenv("MF3D_SWEEP_3D", "1" if tiles == "3d" else "0")
+        for precision in ("fp64", "mixed"):
+            with _gpu(grid, L, lsm, dens, maxcy=ncyc, eps=0.0, precision=precision) as mg:
+                with warnings.catch_warnings():
+                    warnings.simplefilter("ignore", NotConverged)
+                    p_g, res = mg.newsolv_mg(f)
+            assert res.iters == ncyc
+            got[tiles, precision] = p_g
+    for tiles in ("3d", "march"):
+        for a, b in zip(got[tiles, "fp64"], p_o):
+            assert np.array_equal(a, b), f"{name}/{tiles}: p differs, max {np.max(np.abs(a - b))}"
+    for a, b in zip(got["3d", "mixed"], got["march", "mixed"]):
+        assert np.array_equal(a, b), f"{name}: mixed 3-D and march differ, max {np.max(np.abs(a - b))}"
