@@ -126,3 +126,44 @@ def test_both_sweep_kernels_bitwise(name, monkeypatch):
             assert np.array_equal(a, b), f"{name}/{tiles}: p differs, max {np.max(np.abs(a - b))}"
     for a, b in zip(got["3d", "mixed"], got["march", "mixed"]):
         assert np.array_equal(a, b), f"{name}: mixed 3-D and march differ, max {np.max(np.abs(a - b))}"
+
+
+SWITCHES = {
+    "no_tail": {"MF3D_TAIL": "0"},
+    "gather_ghosts": {"MF3D_OWN_GHOSTS": "0"},
+    "spatial_tiles": {"MF3D_XCD_ORDER": "0"},
+    "default_depths": {"MF3D_AUTOTUNE": "0"},
+    "pinned_depth": {"MF3D_SWEEP_TZ": "4"},
+    "all_off": {"MF3D_TAIL": "0", "MF3D_OWN_GHOSTS": "0", "MF3D_XCD_ORDER": "0", "MF3D_AUTOTUNE": "0"},
+}
+
+
+@pytest.mark.parametrize("switch", sorted(SWITCHES))
+@pytest.mark.parametrize("name", ["single32_L5", "neumann_L4", "lsm16_L4", "mixed_bc_multi_L3"])
+def test_layout_switches_keep_results_bitwise(name, switch, monkeypatch):
+    """Each environment switch of the library changes only how the work is
+    laid out (tail kernel, who writes a lone block's ghosts, tile order,
+    chunk depths): FP64 stays bitwise the oracle's."""
+    for k, v in SWITCHES[switch].items():
+        monkeypatch.setenv(k, v)
+    grid, L, lsm = case(name)
+    f, dens = inputs(grid, lsm, seed=13)
+    ncyc = 3
+    p_o, rho_o, _, _ = _oracle(grid, L, lsm, dens).solve(f, 0.0, ncyc)
+    with _gpu(grid, L, lsm, dens, maxcy=ncyc, eps=0.0) as mg:
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", NotConverged)
+            p_g, res = mg.newsolv_mg(f)
+    assert res.iters == ncyc
+    np.testing.assert_allclose(res.hist, rho_o, rtol=1e-12, atol=0)
+    for a, b in zip(p_g, p_o):
+        assert np.array_equal(a, b), f"{name}/{switch}: p differs, max {np.max(np.abs(a - b))}"
+
+
+def test_invalid_pinned_depth_is_rejected(monkeypatch):
+    """MF3D_SWEEP_TZ must be an even depth: an odd one fails the handle's
+    creation with a message, not a silent fallback."""
+    monkeypatch.setenv("MF3D_SWEEP_TZ", "3")
+    grid, L, lsm = case("multi_uneven_L3")
+    with pytest.raises(Exception, match="MF3D_SWEEP_TZ"):
+        PressureMultigrid(grid, L)
