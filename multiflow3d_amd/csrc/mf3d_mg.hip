@@ -1185,15 +1185,16 @@ int build(mf3d_mg *mg) {
     return 0;
 }
 
-// Tile autotuner: for every level the sweep kernel runs on, time the sweep
-// over chunk depths tz from 2 to 64 (the finest level with 8-row tiles, the
+// Tile autotuner: for every level the sweep runs on, time the march over
+// chunk depths tz from 2 to 64 (the finest level with 8-row tiles, the
 // others with 4-row tiles) and keep the fastest.  Each candidate is launched
 // once to warm up and then timed over 3 launches with HIP events on the
 // handle's stream (12 timed launches on levels under 2^20 cells), on the
 // level's own (zeroed) arrays; the fields are zeroed again afterwards.  Of
 // the depths within 3 % of the fastest the deepest is kept (fewer halo
-// planes read).  The one-pass 3-D tile kernel (k_rb3d) is timed too and
-// taken where it beats the best march.  The tiling never changes a result.  MF3D_AUTOTUNE=0 keeps the default depths;
+// planes read).  Levels of at most 2^18 cells take the one-pass 3-D tile
+// kernel (k_rb3d) instead, without timing.  The tiling never changes a
+// result.  MF3D_AUTOTUNE=0 keeps the default depths;
 // MF3D_SWEEP_TZ=<even depth> uses that march depth on every level without
 // timing (to pin the tiling of a profile); MF3D_SWEEP_3D=1 uses the 3-D
 // tiles on every level, =0 never.
@@ -1228,6 +1229,13 @@ int autotune_sweeps(mf3d_mg *mg) {
     HIPCHK(hipEventCreate(&e1));
     // tiles of a choice: a march depth, or kTz3d for the 3-D tiles
     constexpr int kTz3d = -1;
+    // Levels of at most this many cells take the 3-D tiles, larger ones the
+    // march, by measurement inside the cycle: there the 3-D tiles sweep a
+    // 64^3 or 32^3 level in about 6 us against 8-9 us for the best march,
+    // and lose at 128^3 (15 us against 13) and 256^3.  Timing the two here
+    // would not show it: the timing repeats one launch on cache-resident
+    // data, which favours the 3-D tiles' halo re-reads.
+    constexpr long long k3dCells = 1LL << 18;
     auto tiles_of = [&](const Level &V, int ty, int tz) {
         return xcd_order(tz == kTz3d ? tiles_3d(V.hb) : sweep_tiles(V.hb, ty, tz));
     };
@@ -1269,9 +1277,12 @@ int autotune_sweeps(mf3d_mg *mg) {
             auto it = g_tune_tz.find(key);
             if (it != g_tune_tz.end()) choice = it->second;
         }
+        long long cells = 0;
+        for (const BlockLevel &g : V.hb) cells += (long long)g.n0 * g.n1 * g.n2;
         if (force3d == 1) choice = kTz3d;
         else if (pin_tz) choice = pin_tz;
         else if (force3d == 0 && choice == kTz3d) choice = 0;
+        else if (force3d != 0 && cells <= k3dCells) choice = kTz3d;
         if (!choice) {
             int nzmax = 1;
             for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
@@ -1287,7 +1298,6 @@ int autotune_sweeps(mf3d_mg *mg) {
             for (const auto &e : timed) best = std::min(best, e.second);
             for (const auto &e : timed)
                 if (e.second <= 1.03f * best && best < 1e29f) choice = std::max(choice, e.first);
-            if (force3d != 0 && time_tiles(V, tiles_of(V, ty, kTz3d), 0) < best) choice = kTz3d;
             if (choice) {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
                 g_tune_tz[key] = choice;
