@@ -1185,22 +1185,24 @@ int build(mf3d_mg *mg) {
     return 0;
 }
 
-// Tile autotuner: for every level the sweep runs on, time the march over
-// chunk depths tz from 2 to 64 (the finest level with 8-row tiles, the
-// others with 4-row tiles) and keep the fastest.  Each candidate is launched
-// once to warm up and then timed over 3 launches with HIP events on the
-// handle's stream (12 timed launches on levels under 2^20 cells), on the
-// level's own (zeroed) arrays; the fields are zeroed again afterwards.  Of
-// the depths within 3 % of the fastest the deepest is kept (fewer halo
-// planes read).  Levels of at most 2^18 cells take the one-pass 3-D tile
-// kernel (k_rb3d) instead, without timing.  The tiling never changes a
-// result.  MF3D_AUTOTUNE=0 keeps the default depths;
+// Tile autotuner.  On the finest level the march is timed over chunk
+// depths tz from 2 to 64: each candidate is launched once to warm up and
+// then timed over 3 launches (12 on levels under 2^20 cells) with HIP
+// events on the handle's stream, on the level's own (zeroed) arrays, which
+// are zeroed again afterwards; of the depths within 3 % of the fastest the
+// deepest is kept (fewer halo planes read).  Coarser levels keep the
+// default depth (the deepest chunk that still gives about 1024
+// workgroups): inside a cycle it beat the depth such timing picked (128^3:
+// 13 us at 8-plane chunks against 17 at 16), since the timing repeats one
+// launch on cache-resident data.  Levels of at most 2^18 cells take the
+// one-pass 3-D tile kernel (k_rb3d) instead.  The tiling never changes a
+// result.  MF3D_AUTOTUNE=0 keeps the default depths everywhere;
 // MF3D_SWEEP_TZ=<even depth> uses that march depth on every level without
 // timing (to pin the tiling of a profile); MF3D_SWEEP_3D=1 uses the 3-D
 // tiles on every level, =0 never.
-// Chosen depths are remembered per process, keyed by precision, coefficient
-// form, tile rows and the level's block extents, so later handles of the
-// same shape (the serving pool's) skip the timing.
+// The finest level's depth is remembered per process, keyed by precision,
+// coefficient form, tile rows and the level's block extents, so later
+// handles of the same shape (the serving pool's) skip the timing.
 std::mutex g_tune_mu;
 std::map<std::string, int> g_tune_tz;
 
@@ -1232,9 +1234,7 @@ int autotune_sweeps(mf3d_mg *mg) {
     // Levels of at most this many cells take the 3-D tiles, larger ones the
     // march, by measurement inside the cycle: there the 3-D tiles sweep a
     // 64^3 or 32^3 level in about 6 us against 8-9 us for the best march,
-    // and lose at 128^3 (15 us against 13) and 256^3.  Timing the two here
-    // would not show it: the timing repeats one launch on cache-resident
-    // data, which favours the 3-D tiles' halo re-reads.
+    // and lose at 128^3 (15 us against 13) and 256^3.
     constexpr long long k3dCells = 1LL << 18;
     auto tiles_of = [&](const Level &V, int ty, int tz) {
         return xcd_order(tz == kTz3d ? tiles_3d(V.hb) : sweep_tiles(V.hb, ty, tz));
@@ -1281,8 +1281,8 @@ int autotune_sweeps(mf3d_mg *mg) {
         for (const BlockLevel &g : V.hb) cells += (long long)g.n0 * g.n1 * g.n2;
         if (force3d == 1) choice = kTz3d;
         else if (pin_tz) choice = pin_tz;
-        else if (force3d == 0 && choice == kTz3d) choice = 0;
         else if (force3d != 0 && cells <= k3dCells) choice = kTz3d;
+        else if (l > 0) continue;  // coarser march levels keep the default depth
         if (!choice) {
             int nzmax = 1;
             for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
