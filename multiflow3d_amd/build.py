@@ -46,16 +46,19 @@ def build(verbose: bool = False) -> str:
 
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN_DIR = os.path.join(HERE, "fortran")
-FORTRAN_HOST = os.path.join(FORTRAN_DIR, "_build", "mf3d_host")
-FORTRAN_HOST_VEL = os.path.join(FORTRAN_DIR, "_build", "mf3d_host_vel")
+# executables in fortran/bin (they travel with the tree); objects and
+# module files in fortran/_build
+FORTRAN_HOST = os.path.join(FORTRAN_DIR, "bin", "mf3d_host")
+FORTRAN_HOST_VEL = os.path.join(FORTRAN_DIR, "bin", "mf3d_host_vel")
 
 
 def build_fortran(verbose: bool = False) -> str:
     """The ISO_C_BINDING module and the small Fortran hosts that drive the
-    library through it (fortran/_build/mf3d_host for the Poisson form,
+    library through it (fortran/bin/mf3d_host for the Poisson form,
     mf3d_host_vel for the velocity form; linked to libmf3d_mg.so)."""
     out = os.path.join(FORTRAN_DIR, "_build")
     os.makedirs(out, exist_ok=True)
+    os.makedirs(os.path.dirname(FORTRAN_HOST), exist_ok=True)
     steps = [
         [FLANG, "-c", "-O2", "-J", out, os.path.join(FORTRAN_DIR, "mf3d_mg_f.f90"),
          "-o", os.path.join(out, "mf3d_mg_f.o")],

@@ -1240,11 +1240,11 @@ int autotune_sweeps(mf3d_mg *mg) {
         return xcd_order(tz == kTz3d ? tiles_3d(V.hb) : sweep_tiles(V.hb, ty, tz));
     };
     // 1 warm-up + 3 timed sweeps over a candidate tiling (12 on levels under
-    // 2^20 cells, whose few-microsecond sweeps time noisily); ms, 1e30 on failure
-    auto time_tiles = [&](Level &V, const std::vector<Tile> &t, int ty) -> float {
+    // 2^20 cells, whose few-microsecond sweeps time noisily) from the tile
+    // buffer tmp (allocated once per tuning pass, large enough for every
+    // candidate); ms, 1e30 on failure
+    auto time_tiles = [&](Level &V, const std::vector<Tile> &t, int ty, Tile *tmp) -> float {
         const int reps = V.cells_gh < (1LL << 20) ? 12 : 3;
-        Tile *tmp = nullptr;
-        if (hipMalloc((void **)&tmp, sizeof(Tile) * t.size()) != hipSuccess) return 1e30f;
         Tile *keep = V.t_sw;
         const int keep_n = V.n_sw, keep_ty = V.sw_ty;
         V.t_sw = tmp;
@@ -1263,7 +1263,6 @@ int autotune_sweeps(mf3d_mg *mg) {
         V.t_sw = keep;
         V.n_sw = keep_n;
         V.sw_ty = keep_ty;
-        (void)hipFree(tmp);
         return ms;
     };
     int rc = 0;
@@ -1289,10 +1288,23 @@ int autotune_sweeps(mf3d_mg *mg) {
             // powers of two, and depths whose step count (tz + 2) is a whole
             // number of the march's 10-step unrolled blocks
             static const int kTz[] = {2, 4, 8, 16, 18, 28, 32, 38, 48, 58, 64};
-            std::vector<std::pair<int, float>> timed;  // (depth, ms)
+            std::vector<std::pair<int, std::vector<Tile>>> cand;
+            size_t most = 1;
             for (int tz : kTz) {
                 if (tz > std::max(2, nzmax)) break;
-                timed.emplace_back(tz, time_tiles(V, tiles_of(V, ty, tz), ty));
+                cand.emplace_back(tz, tiles_of(V, ty, tz));
+                most = std::max(most, cand.back().second.size());
+            }
+            Tile *tmp = nullptr;
+            if (hipMalloc((void **)&tmp, sizeof(Tile) * most) != hipSuccess) {
+                (void)hipGetLastError();
+                cand.clear();  // untimed: the default depth stays
+            }
+            std::vector<std::pair<int, float>> timed;  // (depth, ms)
+            for (const auto &c : cand) timed.emplace_back(c.first, time_tiles(V, c.second, ty, tmp));
+            if (tmp) {
+                (void)hipStreamSynchronize(mg->stream);
+                (void)hipFree(tmp);
             }
             float best = 1e30f;
             for (const auto &e : timed) best = std::min(best, e.second);

@@ -124,6 +124,10 @@ class Inputs:
                 self.density(r.n)
 
 
+class AdmissionError(ValueError):
+    """A request whose solver handle can never fit the HBM budget."""
+
+
 class SolverPool:
     """Resident handles per (n, two_phase); HBM-budgeted, LRU eviction of idle ones."""
 
@@ -151,6 +155,10 @@ class SolverPool:
             self._lru.pop(id(mg), None)
             return mg
         need = self._bytes(n, two_phase)
+        if need > self.budget:  # would wait forever: nothing to evict frees enough
+            raise AdmissionError(f"a {n}^3 {'two-phase' if two_phase else 'constant-density'} solve "
+                                 f"needs {need} bytes of HBM, more than the pool's whole budget "
+                                 f"of {self.budget} bytes")
         while self.used + need > self.budget and self._lru:
             _, (k, victim) = self._lru.popitem(last=False)
             self.idle[k].remove(victim)
@@ -214,6 +222,11 @@ class Server:
         """Serve `reqs` (sorted by arrival).  With realtime=False every
         request counts as arrived at time 0 (a closed batch)."""
         self.inputs.prepare(reqs)
+        # one handle per shape before the clock starts: its sweep tiling is
+        # timed on an idle device, and handles created later in the run find
+        # that choice in the library's per-process cache instead of timing
+        # while other slots' solves run
+        self.pool.reserve(sorted({(r.n, r.two_phase) for r in reqs}), 1)
         work: List["queue.Queue"] = [queue.Queue() for _ in range(self.slots)]
         done_q: "queue.Queue" = queue.Queue()
         t0 = time.perf_counter()

@@ -1,5 +1,6 @@
 """Host side of the serving loop: request stream and level choice (no GPU)."""
 import numpy as np
+import pytest
 
 from multiflow3d_amd import serving as S
 
@@ -24,3 +25,14 @@ def test_poisson_requests_deterministic_and_rate():
         assert abs(np.mean(sizes == n) - 1 / 6) < 0.03
     assert abs(np.mean([r.two_phase for r in a]) - 0.5) < 0.03
     assert set(r.tol for r in a) == set(S.TOLERANCES)
+
+
+def test_request_larger_than_budget_is_refused():
+    """A handle that cannot fit even an empty pool is refused with an error
+    (waiting for memory to free up would never end), before any device work."""
+    pool = S.SolverPool(S.Inputs(), budget_bytes=1 << 20, precision="mixed", maxcy=20)
+    need = pool._bytes(64, False)
+    assert need > (1 << 20)
+    with pytest.raises(S.AdmissionError, match="more than the pool's whole budget"):
+        pool.acquire(64, False)
+    assert pool.used == 0 and pool.created == 0
