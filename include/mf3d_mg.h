@@ -173,6 +173,29 @@ int mf3d_mg_solve(mf3d_mg *mg, const double *const *f_blocks, double *const *p_b
 int mf3d_mg_solve_device(mf3d_mg *mg, const double *f_dev, double *p_dev, void *stream,
                          double *rmax_out, int32_t *iters_out, double *hist);
 
+/* Allocation-free form of mf3d_mg_solve_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  One call builds the level hierarchy of cfg/blocks in a
+ * fixed device array of MF3D_NOALLOC_SCRATCH_BYTES that the library holds
+ * on each GPU it runs on until the process ends, runs the solve with
+ * kernels launched one by one on `stream` (NULL = the null stream), and
+ * returns when it is complete; it calls no hipMalloc, hipFree or other
+ * allocating routine, and it needs no handle.  f_dev, p_dev, rmax_out,
+ * iters_out and hist are as in mf3d_mg_solve_device (results are bitwise
+ * those of a handle created with the same cfg and blocks, with the default
+ * sweep tiling, MF3D_AUTOTUNE=0).  Constant coefficients only
+ * (variable_coef = 0, else MF3D_ENOTSUP).  A problem whose hierarchy does
+ * not fit the array returns MF3D_ENOMEM before any device work.  The array
+ * is shared by every call on a GPU and keeps the last call's contents
+ * (each call clears what it reads); calls that overlap on one GPU (other
+ * threads, other streams) race and must be serialised by the caller.
+ * err (may be NULL) receives up to err_len bytes of text on failure. */
+#define MF3D_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mf3d_mg_solve_device_noalloc(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, const double *f_dev,
+                                 double *p_dev, void *stream, double *rmax_out, int32_t *iters_out, double *hist,
+                                 char *err, size_t err_len);
+
 /* newsolv_mg (newsolv_mg.for:2-102) in the reference's velocity form, bitwise
  * the reference in FP64:
  *

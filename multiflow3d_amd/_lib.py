@@ -24,8 +24,9 @@ EXPORTS = (
     "mf3d_mg_graph_replay", "mf3d_mg_set_eps", "mf3d_mg_newsolv", "mf3d_mg_newsolv_device",
     "mf3d_mg_ghosted_cells", "mf3d_mg_set_density_device", "mf3d_mg_heaviside",
     "mf3d_mg_heaviside_device", "mf3d_tecbin_header", "mf3d_tecbin_read", "mf3d_final_ctime_read",
-    "mf3d_io_last_error",
+    "mf3d_io_last_error", "mf3d_mg_solve_device_noalloc",
 )
+MF3D_NOALLOC_SCRATCH_BYTES = 32 << 20
 MF3D_ENOENT, MF3D_EFORMAT = -2, -74
 MF3D_COARSE_REFERENCE, MF3D_COARSE_FACE = 0, 1
 MF3D_OUTER_DEFECT, MF3D_OUTER_GCR = 0, 1
@@ -128,6 +129,10 @@ def lib():
     L.mf3d_final_ctime_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(FinalCtime)]
     L.mf3d_io_last_error.restype = ctypes.c_char_p
     L.mf3d_io_last_error.argtypes = []
+    L.mf3d_mg_solve_device_noalloc.restype = ctypes.c_int
+    L.mf3d_mg_solve_device_noalloc.argtypes = [ctypes.POINTER(Cfg), ctypes.POINTER(BlockDesc), vp, vp, vp, dp,
+                                               ctypes.POINTER(ctypes.c_int32), dp, ctypes.c_char_p,
+                                               ctypes.c_size_t]
     L.mf3d_mg_heaviside_device.restype = ctypes.c_int
     L.mf3d_mg_heaviside_device.argtypes = [vp, vp, d, d, d, d, vp, vp, vp]
     _lib = L

@@ -43,6 +43,13 @@
 
 using namespace mf3d;
 
+// Device memory of the allocation-free solve (mf3d_mg_solve_device_noalloc):
+// one array per GPU, loaded with the library and held until the process
+// ends.  Host code never uses its name as an address: each device's copy
+// is found once with hipGetSymbolAddress (scratch_base) and that pointer
+// is what kernels and copies receive.
+static __device__ __align__(256) unsigned char scratch[MF3D_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int kTileX = 64, kTileY = 4;
@@ -158,6 +165,11 @@ struct mf3d_mg {
     // ADI smoother: a(m) scratch of the per-pass kernels (level-0 size,
     // shared by the levels whose blocks do not fit in LDS)
     void *d_adi = nullptr;
+    // allocation-free solve (mf3d_mg_solve_device_noalloc): every device
+    // buffer is carved from this device's `scratch` array instead of
+    // hipMalloc; no graphs, no tuner, no staging, the caller's stream
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
 
 namespace {
@@ -174,6 +186,18 @@ namespace {
 int dev_alloc(mf3d_mg *mg, void **p, size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
     if (bytes == 0) bytes = 256;
+    if (mg->arena) {  // carve from the scratch array (256-byte aligned)
+        if (bytes > mg->arena_cap - mg->arena_used) {
+            mg->err = "the solve needs more device memory than the " +
+                      std::to_string(mg->arena_cap >> 20) + " MiB scratch of mf3d_mg_solve_device_noalloc";
+            *p = nullptr;
+            return MF3D_ENOMEM;
+        }
+        *p = mg->arena + mg->arena_used;
+        mg->arena_used += bytes;
+        mg->bytes += (long long)bytes;
+        return 0;
+    }
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         mg->err = std::string("hipMalloc: ") + hipGetErrorString(e);
@@ -766,14 +790,16 @@ int solve_dev(mf3d_mg *mg, const double *f_packed, double *p_packed, double *rma
     Level &L0 = mg->lv[0];
     const int maxcy = mg->cfg.maxcy;
     int chunk = mg->guess > 0 ? mg->guess : 4;
-    const bool use_graph = mg->tk == MF3D_K_OFF;
+    // the allocation-free solve launches its kernels one by one
+    // (instantiating a graph may allocate)
+    const bool use_graph = mg->tk == MF3D_K_OFF && !mg->arena;
     if (use_graph && !mg->graphs_ready) {
         int rc = capture_cycles<T, VAR>(mg);
         if (rc) return rc;
     }
     // timed graphs for the first chunk, captured while the stream is idle
     if (mg->tk_eager) drop_timed_graphs(mg);
-    bool timed = !use_graph && !mg->tk_eager;
+    bool timed = mg->tk != MF3D_K_OFF && !mg->tk_eager;
     for (int k = 1; timed && k <= std::min(chunk, maxcy); ++k)
         if (capture_timed<T, VAR>(mg, k)) {
             drop_timed_graphs(mg);
@@ -1139,9 +1165,11 @@ int build(mf3d_mg *mg) {
         (rc = dev_alloc(mg, (void **)&mg->d_hist, sizeof(double) * (c.maxcy + 1))) ||
         (rc = dev_alloc(mg, (void **)&mg->d_rmax, sizeof(double) * (c.maxcy + 1))))
         return rc;
-    mg->io_len = std::max(L0.cells_gh, mg->cells);
-    if ((rc = dev_alloc(mg, (void **)&mg->d_io, sizeof(double) * mg->io_len))) return rc;
-    HIPCHK(hipHostMalloc((void **)&mg->h_io, sizeof(double) * mg->io_len, hipHostMallocDefault));
+    if (!mg->arena) {  // host-array entries' staging (the allocation-free solve has none)
+        mg->io_len = std::max(L0.cells_gh, mg->cells);
+        if ((rc = dev_alloc(mg, (void **)&mg->d_io, sizeof(double) * mg->io_len))) return rc;
+        HIPCHK(hipHostMalloc((void **)&mg->h_io, sizeof(double) * mg->io_len, hipHostMallocDefault));
+    }
     // every field starts at zero (coeff zeroes phi, mgsolver.for:60-67)
     for (int l = 0; l < L; ++l) {
         Level &V = mg->lv[l];
@@ -1179,6 +1207,7 @@ int build(mf3d_mg *mg) {
             return rc;
     }
     if (c.smoother != 3) return 0;  // the tiles tuned below belong to the red-black sweep
+    if (mg->arena) return 0;        // the tuner allocates: default tilings
     if ((rc = mixed ? (c.variable_coef ? autotune_sweeps<float, true>(mg) : autotune_sweeps<float, false>(mg))
                     : (c.variable_coef ? autotune_sweeps<double, true>(mg) : autotune_sweeps<double, false>(mg))))
         return rc;
@@ -1376,6 +1405,29 @@ int build_lsm(mf3d_mg *mg) {
 // At every entry point: select the device and drop an error left in this
 // thread's HIP state by unrelated earlier calls, so HIPCHK(hipGetLastError())
 // below reports only this call's own failures.
+// This device's copy of `scratch`, looked up once per device (with the
+// device current) and kept for the process.
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;
+
+int scratch_base(int dev, void **out, std::string *err) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return 0;
+    }
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return MF3D_EDEVICE;
+    }
+    g_scratch_ptr[dev] = ptr;
+    *out = ptr;
+    return 0;
+}
+
 int set_device(mf3d_mg *mg) {
     (void)hipGetLastError();
     HIPCHK(hipSetDevice(mg->cfg.device));
@@ -2186,6 +2238,59 @@ int mf3d_mg_solve_device(mf3d_mg *mg, const double *f_dev, double *p_dev, void *
     rc = solve_any(mg, f_dev, p_dev, rmax_out, iters_out, hist);
     mg->stream = own;
     return rc;
+}
+
+int mf3d_mg_solve_device_noalloc(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, const double *f_dev,
+                                 double *p_dev, void *stream, double *rmax_out, int32_t *iters_out, double *hist,
+                                 char *err, size_t err_len) {
+    mf3d_mg tmp;  // host-side state of this call only
+    auto fail = [&](int rc) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", tmp.err.c_str());
+        return rc;
+    };
+    if (err && err_len) err[0] = '\0';
+    if (!cfg || !blocks || !f_dev || !p_dev) {
+        tmp.err = "null argument";
+        return fail(MF3D_EINVAL);
+    }
+    std::string v = validate(*cfg, blocks);
+    if (!v.empty()) {
+        tmp.err = v;
+        return fail(MF3D_EINVAL);
+    }
+    if (cfg->variable_coef) {
+        tmp.err = "mf3d_mg_solve_device_noalloc solves constant-coefficient problems only (a density "
+                  "needs a handle: mf3d_mg_set_density)";
+        return fail(MF3D_ENOTSUP);
+    }
+    // checked before any device work: the estimate less the host entries'
+    // staging (which this solve does not hold), plus room for the tables
+    long long gh0 = 0, in0 = 0;
+    for (int i = 0; i < cfg->nblocks; ++i) {
+        gh0 += (long long)(blocks[i].n[0] + 2) * (blocks[i].n[1] + 2) * (blocks[i].n[2] + 2);
+        in0 += (long long)blocks[i].n[0] * blocks[i].n[1] * blocks[i].n[2];
+    }
+    const long long need = estimate(*cfg, blocks) - std::max(gh0, in0) * 8 + (1LL << 20);
+    if (need > (long long)MF3D_NOALLOC_SCRATCH_BYTES) {
+        tmp.err = "the solve needs about " + std::to_string(need >> 20) + " MiB of device memory, more than the " +
+                  std::to_string((long long)MF3D_NOALLOC_SCRATCH_BYTES >> 20) +
+                  " MiB scratch of mf3d_mg_solve_device_noalloc";
+        return fail(MF3D_ENOMEM);
+    }
+    tmp.cfg = *cfg;
+    tmp.blk.assign(blocks, blocks + cfg->nblocks);
+    int rc = set_device(&tmp);
+    if (rc) return fail(rc);
+    void *base = nullptr;
+    if ((rc = scratch_base(cfg->device, &base, &tmp.err))) return fail(rc);
+    tmp.arena = (char *)base;
+    tmp.arena_cap = MF3D_NOALLOC_SCRATCH_BYTES;
+    tmp.stream = (hipStream_t)stream;  // NULL: the null stream
+    if ((rc = build(&tmp))) return fail(rc);
+    rc = solve_any(&tmp, f_dev, p_dev, rmax_out, iters_out, hist);
+    (void)hipStreamSynchronize(tmp.stream);
+    tmp.stream = nullptr;  // not ours to destroy
+    return rc < 0 ? fail(rc) : rc;
 }
 
 int mf3d_mg_newsolv_device(mf3d_mg *mg, const double *ustar, const double *vstar, const double *wstar, double *u,

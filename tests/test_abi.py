@@ -132,3 +132,22 @@ def test_sphere_density_recipe():
     assert rho.min() == 1.0 and rho.max() == 1000.0
     assert rho[16, 16, 16] == 1.0 and rho[0, 0, 0] == 1000.0
     assert W.heaviside(np.array([0.0]), 0.1)[0] == 0.5
+
+
+def test_noalloc_solve_rejects_before_device_work():
+    """The allocation-free solve refuses a hierarchy larger than its fixed
+    scratch array, and a variable-coefficient problem, with a message and
+    before any device call (the pointers are never dereferenced)."""
+    from multiflow3d_amd.solver import _descs, make_cfg
+    L = _lib.lib()
+    cases = ((T.single_block([128] * 3, [1 / 128] * 3, T.CHANNEL_BC), {}, _lib.MF3D_ENOMEM, "scratch"),
+             (T.single_block([16] * 3, [1 / 16] * 3, T.CHANNEL_BC), {"variable_coef": True},
+              _lib.MF3D_ENOTSUP, "constant-coefficient"))
+    for grid, kw, code, text in cases:
+        cfg = make_cfg(1, 4, **kw)
+        err = ctypes.create_string_buffer(256)
+        rc = L.mf3d_mg_solve_device_noalloc(ctypes.byref(cfg), _descs(grid.blocks, 1), ctypes.c_void_p(8),
+                                            ctypes.c_void_p(8), None, None, None, None, err, 256)
+        assert rc == code, (rc, err.value)
+        assert text in err.value.decode()
+    assert _lib.MF3D_NOALLOC_SCRATCH_BYTES == 32 << 20
