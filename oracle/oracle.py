@@ -259,10 +259,11 @@ def have_ref_vel() -> bool:
 
 
 def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None, nu_pre=2, nu_post=2,
-                  smoother=3, cwd=None):
+                  smoother=3, cwd=None, pl_ex=0, lmr=1, differencing=1):
     """One run of mg_ref_vel in `mode` (0 constant, 1 density, 2 level set,
     3 bound_lsm only, 4 constant + checkpoint files written into cwd) on
-    the concatenated input `parts`; returns its raw f64 output."""
+    the concatenated input `parts` (arrays with lmr + pl_ex ghost layers);
+    returns its raw f64 output."""
     if not have_ref_vel():
         raise RuntimeError("compiled reference not built (make -C oracle ref)")
     with tempfile.TemporaryDirectory() as td:
@@ -273,7 +274,7 @@ def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None,
         with open(ctl, "w") as fh:
             nbx, nby, nbz = grid.shape
             fh.write(f"{nbx} {nby} {nbz} {levels} {mode} {maxcy} {nu_pre} {nu_post}\n")
-            fh.write(f"{eps!r} {smoother}\n")
+            fh.write(f"{eps!r} {smoother} {pl_ex} {lmr} {differencing}\n")
             fh.write(" ".join(str(c) for c in grid.bc) + "\n")
             fh.write(" ".join(map(str, grid.ex)) + "\n")
             fh.write(" ".join(map(str, grid.ey)) + "\n")
@@ -287,27 +288,29 @@ def _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props=None,
         return np.fromfile(fout, dtype=np.float64)
 
 
-def _boxes(grid, raw, o, keys):
+def _boxes(grid, raw, o, keys, G=1):
     out = {key: [] for key in keys}
     for blk in grid.blocks:
         nx, ny, nz = blk.n
-        m = (nx + 2) * (ny + 2) * (nz + 2)
+        m = (nx + 2 * G) * (ny + 2 * G) * (nz + 2 * G)
         for key in keys:
-            out[key].append(raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy())
+            out[key].append(raw[o:o + m].reshape(nz + 2 * G, ny + 2 * G, nx + 2 * G).copy())
             o += m
     return out, o
 
 
 def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: int, dt: float,
                 alfapr: float, lsm: bool = False, dens=None, nu_pre: int = 2, nu_post: int = 2,
-                phi=None, props=None, smoother: int = 3):
+                phi=None, props=None, smoother: int = 3, pl_ex: int = 0, lmr: int = 1,
+                differencing: int = 1):
     """Run the compiled reference's velocity-form newsolv_mg loop (calvel,
     calmas, mgkcyc, p update, exchange(4), pbound).  Every field is a list of
-    full ghosted block arrays (one ghost layer), shaped (z+2, y+2, x+2).
-    Returns (iterations, rmax history, p, u, v, w, su) with the fields as
-    lists of ghosted block arrays.  With a level set `phi` and
-    props = (densl, densg, mul, mug) the reference's heaviside runs first
-    (lsm.for:1456-1509) and dens and mu are returned as well."""
+    full ghosted block arrays with G = lmr + pl_ex ghost layers (the
+    reference's pl), shaped (z+2G, y+2G, x+2G).  Returns (iterations, rmax
+    history, p, u, v, w, su) with the fields as lists of ghosted block
+    arrays.  With a level set `phi` and props = (densl, densg, mul, mug)
+    the reference's heaviside runs first (lsm.for:1456-1509) and dens and
+    mu are returned as well."""
     mode = 2 if phi is not None else int(lsm)
     parts = []
     for b in range(len(grid.blocks)):
@@ -316,12 +319,13 @@ def run_ref_vel(grid, levels: int, ustar, vstar, wstar, p0, eps: float, maxcy: i
             parts.append(dens[b])
         elif mode == 2:
             parts.append(phi[b])
-    raw = _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props, nu_pre, nu_post, smoother)
+    raw = _ref_vel_call(grid, levels, mode, parts, maxcy, eps, dt, alfapr, props, nu_pre, nu_post, smoother,
+                        pl_ex=pl_ex, lmr=lmr, differencing=differencing)
     k = int(raw[0])
     o = 1
     rmh = raw[o:o + maxcy + 1].copy(); o += maxcy + 1
     keys = ("p", "u", "v", "w", "su") + (("dens", "mu") if mode == 2 else ())
-    out, o = _boxes(grid, raw, o, keys)
+    out, o = _boxes(grid, raw, o, keys, G=lmr + pl_ex)
     res = (k, rmh[:k + 1], out["p"], out["u"], out["v"], out["w"], out["su"])
     return res + (out["dens"], out["mu"]) if mode == 2 else res
 

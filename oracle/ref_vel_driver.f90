@@ -21,14 +21,16 @@
 !
 ! Usage: mg_ref_vel <control file>
 !   line 1: nbx nby nbz levels lsm maxcy nu_pre nu_post
-!   line 2: eps (absolute: rmax = max|div u| dV, newsolv_mg.for:56) and
-!           mg_itrsch (1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS)
+!   line 2: eps (absolute: rmax = max|div u| dV, newsolv_mg.for:56),
+!           mg_itrsch (1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS),
+!           pl_ex, LMR and differencing (control.cin; the ghost width is
+!           pl = LMR + pl_ex, localparameters.for:17-24)
 !   line 3: bc_w bc_e bc_s bc_n bc_b bc_t
 !   lines 4-6: block extents along x, y, z
 !   line 7: dx dy dz
 !   line 8: dt alfapr
 !   (lsm = 2 only) densl densg mul mug
-!   line 9: input file: per block, full ghosted arrays (pl = 1) of ustar,
+!   line 9: input file: per block, full ghosted arrays (pl layers) of ustar,
 !           vstar, wstar, p and dens (lsm = 1) or phi (lsm = 2), in that
 !           order; lsm = 3: per block the field alone
 !   line 10: output file
@@ -53,7 +55,7 @@ program mg_ref_vel_driver
   call get_command_argument(1, ctl)
   open(newunit=u, file=trim(ctl), status='old')
   read(u, *) nbx, nby, nbz, ngrid_input, lsmf, maxcy, irestr, iproln
-  read(u, *) eps, mg_itrsch
+  read(u, *) eps, mg_itrsch, pl_ex, LMR, differencing
   read(u, *) bcs
   allocate(ex(nbx), ey(nby), ez(nbz))
   read(u, *) ex
@@ -71,10 +73,7 @@ program mg_ref_vel_driver
   ngrd_gl = ngrid_input
   L_LSM = (lsmf == 1 .or. lsmf == 2)
   ntime = 0; ctime = 0.0; forcn = 0.0; qstpn = 0.0; count = 0; ntav1_count = 0.0; ntav2_count = 0.0
-  pl = 1
-  pl_ex = 0
-  differencing = 1
-  LMR = 1
+  pl = LMR + pl_ex
   rdivmax = 1
   g_dx = gdx; g_dy = gdy; g_dz = gdz
   allocate(dom(nbp), dom_id(nbp), dom_ad(0:num_domains-1), dom_indid(0:num_domains-1))

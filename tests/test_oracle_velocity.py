@@ -194,3 +194,113 @@ def test_velocity_oracle_matches_compiled_reference_randomised():
                 assert np.array_equal(a, b), (trial, i)
         for a, b in zip(r[6], o[6]):
             assert np.array_equal(a[1:-1, 1:-1, 1:-1], b)
+
+
+MAN_PL = json.load(open(os.path.join(GOLDEN, "manifest_vel_pl.json")))
+
+
+def load_case_pl(name):
+    """(grid, manifest entry, inputs, expected) of a wide-shell golden case
+    (pl = LMR + pl_ex ghost layers)."""
+    import importlib.util
+    m = MAN_PL[name]
+    g = _grid(m)
+    spec = importlib.util.spec_from_file_location("mkv", os.path.join(GOLDEN, "make_golden_vel.py"))
+    mkv = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mkv)
+    G = m["ghost"]
+    us, vs, ws, p0 = mkv.inputs_pl(g, m["seed"], G)
+    z = np.load(os.path.join(GOLDEN, f"vel_{name}.npz"))
+    exp = {k: _split(g, z[k], ghost=G) for k in ("p", "u", "v", "w")}
+    exp["su"] = _split(g, z["su"], ghost=0)
+    exp["iters"] = int(z["iters"][0])
+    exp["rmax"] = z["rmax"]
+    return g, m, dict(ustar=us, vstar=vs, wstar=ws, p0=p0), exp
+
+
+@pytest.mark.parametrize("name", sorted(MAN_PL))
+def test_wide_shell_oracle_reproduces_reference_fixture(name):
+    """pl_ex extra layers refreshed by exchange(1..3), whole-plane periodic
+    copies and (pl+1)-deep corner/edge cubes reaching the outer layers, and
+    the 4th-order calmas: the oracle's whole ghosted p, u, v, w, su and
+    rmax history equal the compiled reference's bitwise."""
+    g, m, inp, exp = load_case_pl(name)
+    k, hist, p, u, v, w, su = V.newsolv_velocity(
+        g, m["levels"], inp["ustar"], inp["vstar"], inp["wstar"], inp["p0"], m["eps"], m["maxcy"], m["dt"],
+        m["alfapr"], pl_ex=m["pl_ex"], differencing=m["differencing"])
+    assert k == exp["iters"]
+    assert np.array_equal(hist, exp["rmax"])
+    for key, got in (("p", p), ("u", u), ("v", v), ("w", w), ("su", su)):
+        for a, b in zip(got, exp[key]):
+            assert np.array_equal(a, b), (name, key)
+
+
+def test_wide_shell_exchange_depths():
+    """On the shipped deck's shell (pl = 4, pl_ex = 2): entries beyond the
+    first ghost layer change (whole-plane periodic copies, 5-deep corner
+    cubes), so a one-layer refresh fails; the x-face ghost planes of u at
+    depth pl_ex + 2 keep their input values on the interior rows, so a
+    refresh of all pl layers fails too (rows 2..n-1: the corner and edge
+    cubes also reach rows 1 and n)."""
+    g, m, inp, exp = load_case_pl("deck_channel222_pl4")
+    G = m["ghost"]
+    for key, src in (("p", inp["p0"]), ("u", inp["ustar"])):
+        beyond = np.ones(exp[key][0].shape, dtype=bool)
+        beyond[G - 1:-(G - 1), G - 1:-(G - 1), G - 1:-(G - 1)] = False
+        changed = sum(int(np.count_nonzero((a != b)[beyond])) for a, b in zip(exp[key], src))
+        assert changed > 0, key
+    d = m["pl_ex"] + 2  # first layer the exchange does not refresh
+    for a, b, blk in zip(exp["u"], inp["ustar"], g.blocks):
+        nx, ny, nz = blk.n
+        rows = (slice(G + 1, G + nz - 1), slice(G + 1, G + ny - 1))
+        assert np.array_equal(a[rows + (G - d,)], b[rows + (G - d,)])
+        assert np.array_equal(a[rows + (G + nx - 1 + d,)], b[rows + (G + nx - 1 + d,)])
+
+
+def test_fourth_order_calmas_exact_on_cubic_faces():
+    """The 4th-order staggered divergence is exact for face velocities that
+    are cubic along their own axis: u(x) = x^3 at faces x_i = i dx gives
+    du/dx = 3 x^2 at the cell centres (i - 1/2) dx, and likewise for v, w."""
+    n, G = 6, 2
+    g = T.block_grid([n], [n], [n], [T.PERIODIC] * 6, [0.1, 0.2, 0.25])
+    c = np.arange(1 - G, n + G + 1, dtype=np.float64)  # coordinates of the array indices
+    shp = (n + 2 * G,) * 3
+    dx, dy, dz = g.blocks[0].h
+    u = np.broadcast_to((c * dx) ** 3, shp).copy()
+    v = np.broadcast_to(((c * dy) ** 3)[None, :, None], shp).copy()
+    w = np.broadcast_to(((c * dz) ** 3)[:, None, None], shp).copy()
+    su, _ = V.calmas(g, [u], [v], [w], dt=1.0, differencing=2)
+    ctr = c[G:G + n] - 0.5
+    expect = 3 * (ctr[None, None, :] * dx) ** 2 + 3 * (ctr[None, :, None] * dy) ** 2 + 3 * (ctr[:, None, None] * dz) ** 2
+    np.testing.assert_allclose(su[0], expect, rtol=1e-12, atol=1e-13)
+    su2, _ = V.calmas(g, [u], [v], [w], dt=1.0)  # the 2nd-order branch is not exact on cubics
+    assert not np.allclose(su2[0], expect, rtol=1e-6)
+
+
+@pytest.mark.skipif(not have_ref_vel(), reason="oracle/_ref/mg_ref_vel not built (build container only)")
+def test_wide_shell_oracle_matches_compiled_reference_randomised():
+    """Randomised layouts, boundary codes, pl_ex in 0..2, LMR 1 or 2 and
+    the 4th-order calmas where two layers exist, run live against the
+    compiled reference: whole ghosted fields bitwise."""
+    rng = np.random.default_rng(77)
+    for trial in range(6):
+        pl_ex, lmr = int(rng.integers(0, 3)), int(rng.integers(1, 3))
+        G = pl_ex + lmr
+        diff = 2 if G >= 2 and trial % 2 else 1
+        nb = [int(x) for x in rng.integers(1, 3, size=3)]
+        ext = [[int(rng.choice([4, 8])) for _ in range(n)] for n in nb]
+        bc = []
+        for a in range(3):
+            cc = int(rng.choice([T.PERIODIC, T.NEUMANN]))
+            bc += [cc, cc]
+        g = T.block_grid(*ext, bc, [1 / 8] * 3)
+        shp = [(b.n[2] + 2 * G, b.n[1] + 2 * G, b.n[0] + 2 * G) for b in g.blocks]
+        us, vs, ws, p0 = ([rng.uniform(-1, 1, s) for s in shp] for _ in range(4))
+        r = run_ref_vel(g, 2, us, vs, ws, p0, 0.0, 3, 0.01, 1.0, pl_ex=pl_ex, lmr=lmr, differencing=diff)
+        o = V.newsolv_velocity(g, 2, us, vs, ws, p0, 0.0, 3, 0.01, 1.0, pl_ex=pl_ex, differencing=diff)
+        assert r[0] == o[0] and np.array_equal(r[1], o[1][:len(r[1])])
+        for i in (2, 3, 4, 5):
+            for a, b in zip(r[i], o[i]):
+                assert np.array_equal(a, b), (trial, i)
+        for a, b in zip(r[6], o[6]):
+            assert np.array_equal(a[G:-G, G:-G, G:-G], b)
