@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MF3D_MG_ABI_VERSION 5
+#define MF3D_MG_ABI_VERSION 6
 
 enum {
     MF3D_OK = 0,
@@ -73,6 +73,20 @@ enum { MF3D_COARSE_REFERENCE = 0, MF3D_COARSE_FACE = 1 };
  * reference's non-symmetric multi-block density operator; no counterpart in
  * the reference). */
 enum { MF3D_OUTER_DEFECT = 0, MF3D_OUTER_GCR = 1 };
+
+/* Coarsest level (mf3d_mg_cfg.coarse_solve).  The reference solves it by
+ * irestr + iproln smoothing sweeps (mgsolver.for:224-369).
+ * MF3D_COARSEST_DEFAULT: those sweeps, computed in FP64 in both precisions
+ * (the mixed form widens the coarse RHS, sweeps in FP64 and narrows the
+ * result), so the mixed form follows the FP64 form's path.
+ * MF3D_COARSEST_INNER: the sweeps in the inner precision (FP32 in mixed).
+ * MF3D_COARSEST_DIRECT: an exact FP64 solve (a dense inverse of the
+ * coarsest operator, rebuilt with the coefficients, applied every cycle);
+ * it removes the smooth error the sweeps leave, so a solve stops at a
+ * different (more accurate) iterate than the reference's path.  It needs
+ * levels >= 2 and at most MF3D_COARSEST_MAX_CELLS coarsest cells. */
+enum { MF3D_COARSEST_DEFAULT = 0, MF3D_COARSEST_INNER = 1, MF3D_COARSEST_DIRECT = 2 };
+#define MF3D_COARSEST_MAX_CELLS 4096
 
 typedef struct mf3d_mg mf3d_mg;
 
@@ -106,6 +120,11 @@ typedef struct {
     int32_t device;        /* HIP device ordinal */
     int32_t coarse_op;     /* MF3D_COARSE_*; variable-coefficient form only */
     int32_t krylov;        /* MF3D_OUTER_* */
+    int32_t coarse_solve;  /* MF3D_COARSEST_* */
+    int32_t differencing;  /* the reference's differencing (control.cin): 2 = 4th-order calmas
+                              (press.for:13-36, needs ghost >= 2), any other value 2nd order */
+    int32_t pl_ex;         /* extra ghost layers exchange(1..3) refreshes for u, v, w
+                              (exchange.for:14-23); needs ghost >= pl_ex + 1 */
 } mf3d_mg_cfg;
 
 /* coeff sizing + constant coefficients (mgsolver.for:2-222, localparameters.for:200-237). */
@@ -117,8 +136,8 @@ int mf3d_mg_set_density(mf3d_mg *mg, const double *const *dens_blocks);
 
 /* The same from a device-resident density: every block's ghosted array
  * ((n0+2)(n1+2)(n2+2) values, i fastest) concatenated in block order,
- * mf3d_mg_ghosted_cells values (the layout of mf3d_mg_newsolv_device);
- * the ghost layer is kept for the velocity form.  Runs on `stream` (NULL =
+ * mf3d_mg_ghosted_cells values (the layout of mf3d_mg_newsolv_device
+ * when ghost is 1); the ghost layer is kept for the velocity form.  Runs on `stream` (NULL =
  * the handle's own) and returns when the hierarchy is built. */
 int mf3d_mg_set_density_device(mf3d_mg *mg, const double *dens, void *stream);
 
@@ -140,8 +159,8 @@ int mf3d_mg_set_density_device(mf3d_mg *mg, const double *dens, void *stream);
  * level-0 density (with its ghost layer) and coefficient hierarchy are
  * replaced, as by mf3d_mg_set_density.  Needs variable_coef.
  *
- * Device form: phi, dens and mu are ghosted device arrays in the layout of
- * mf3d_mg_newsolv_device; dens and mu may be NULL (not written).  Runs on
+ * Device form: phi, dens and mu are one-layer ghosted device arrays
+ * (mf3d_mg_ghosted_cells values); dens and mu may be NULL (not written).  Runs on
  * `stream` (NULL = the handle's own) and returns when the hierarchy is
  * built.  Host form: caller arrays (ghost >= 1); dens and mu (may be NULL)
  * are written over the interior and the first ghost layer.  sin is the
@@ -210,11 +229,15 @@ int mf3d_mg_solve_device_noalloc(const mf3d_mg_cfg *cfg, const mf3d_block_desc *
  * ustar, vstar, wstar (dom(ib)%ustar ...) are read; u, v, w and p are read
  * and written in place: calvel writes the faces isu..ieu (a face on a
  * physical non-periodic wall keeps the caller's value) and the exchanges
- * and pbound write the one-cell ghost layer exactly as the reference's
- * copy sequence does (ghost layers beyond the first, which the reference
- * refreshes when pl_ex > 0, are not touched).  Caller arrays need
- * ghost >= 1; a face array's index i is the face east of (north of,
- * above) cell i.  eps is the reference's absolute criterion on
+ * and pbound write the ghost shell exactly as the reference's copy
+ * sequence does on arrays with `ghost` = pl layers (every block the same
+ * width): exchange(1..3) over the layers ly = 0..pl_ex, exchange(4) over
+ * ly = 0, whole ghosted planes across periodic faces, (pl+1)-deep corner
+ * and edge cubes (exchange.for:2-341, exchange_bc.for:2-841).  With
+ * differencing = 2, calmas is the 4th-order branch (press.for:13-36) and
+ * needs ghost >= 2.  Caller arrays need ghost >= pl_ex + 1; a face array's
+ * index i is the face east of (north of, above) cell i.  eps is the
+ * reference's absolute criterion on
  * rmax = max|div u|*dx*dy*dz (the eps of control.cin), not mf3d_mg_cfg.eps.
  * Under LSM (variable_coef) mf3d_mg_set_density must have received the
  * cell density with its ghost layer; newsolv_mg.for:12-14 rebuilds the
@@ -230,18 +253,24 @@ int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const
                     double *rmax_out, int32_t *iters_out, double *rmax_hist);
 
 /* The same with device-resident fields: device pointers to every block's
- * ghosted array ((n0+2)(n1+2)(n2+2) values, i fastest) concatenated in
- * block order, mf3d_mg_ghosted_cells values each (su likewise, interior
- * written; may be NULL).  Runs on `stream` (NULL = the handle's own) and
- * returns when the solve is complete.  The captured outer step is kept
- * for the last set of field pointers. */
+ * array with its `ghost` layers ((n0+2g)(n1+2g)(n2+2g) values, i fastest)
+ * concatenated in block order, mf3d_mg_shell_cells values each (su
+ * likewise, interior written; may be NULL).  Runs on `stream` (NULL = the
+ * handle's own) and returns when the solve is complete.  The captured
+ * outer step is kept for the last set of field pointers. */
 int mf3d_mg_newsolv_device(mf3d_mg *mg, const double *ustar, const double *vstar, const double *wstar,
                            double *u, double *v, double *w, double *p, double dt, double alfapr,
                            double eps, double *su, void *stream, double *rmax_out, int32_t *iters_out,
                            double *rmax_hist);
 
-/* Length of the ghosted device fields of mf3d_mg_newsolv_device. */
+/* Length of the one-layer ghosted device fields ((n0+2)(n1+2)(n2+2) per
+ * block) of mf3d_mg_set_density_device and mf3d_mg_heaviside_device. */
 int64_t mf3d_mg_ghosted_cells(const mf3d_mg *mg);
+
+/* Length of the device fields of mf3d_mg_newsolv_device: every block's
+ * array with the blocks' `ghost` layers (= mf3d_mg_ghosted_cells when
+ * ghost is 1). */
+int64_t mf3d_mg_shell_cells(const mf3d_mg *mg);
 
 /* Interior cells of all blocks (length of the device-resident vectors). */
 int64_t mf3d_mg_cells(const mf3d_mg *mg);

@@ -24,12 +24,14 @@ EXPORTS = (
     "mf3d_mg_graph_replay", "mf3d_mg_set_eps", "mf3d_mg_newsolv", "mf3d_mg_newsolv_device",
     "mf3d_mg_ghosted_cells", "mf3d_mg_set_density_device", "mf3d_mg_heaviside",
     "mf3d_mg_heaviside_device", "mf3d_tecbin_header", "mf3d_tecbin_read", "mf3d_final_ctime_read",
-    "mf3d_io_last_error", "mf3d_mg_solve_device_noalloc",
+    "mf3d_io_last_error", "mf3d_mg_solve_device_noalloc", "mf3d_mg_shell_cells",
 )
 MF3D_NOALLOC_SCRATCH_BYTES = 32 << 20
 MF3D_ENOENT, MF3D_EFORMAT = -2, -74
 MF3D_COARSE_REFERENCE, MF3D_COARSE_FACE = 0, 1
 MF3D_OUTER_DEFECT, MF3D_OUTER_GCR = 0, 1
+MF3D_COARSEST_DEFAULT, MF3D_COARSEST_INNER, MF3D_COARSEST_DIRECT = 0, 1, 2
+MF3D_COARSEST_MAX_CELLS = 4096
 MF3D_K_OFF, MF3D_K_SMOOTH, MF3D_K_OUTER, MF3D_K_RESTRICT, MF3D_K_PROLONG, MF3D_K_HEAVISIDE = 0, 1, 2, 3, 4, 5
 
 
@@ -57,7 +59,8 @@ class Cfg(ctypes.Structure):
                 ("eps", ctypes.c_double), ("min_cycles", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("variable_coef", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("coarse_op", ctypes.c_int32),
-                ("krylov", ctypes.c_int32)]
+                ("krylov", ctypes.c_int32), ("coarse_solve", ctypes.c_int32),
+                ("differencing", ctypes.c_int32), ("pl_ex", ctypes.c_int32)]
 
 
 _lib = None
@@ -117,6 +120,8 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int32), dp]
     L.mf3d_mg_ghosted_cells.restype = ctypes.c_int64
     L.mf3d_mg_ghosted_cells.argtypes = [vp]
+    L.mf3d_mg_shell_cells.restype = ctypes.c_int64
+    L.mf3d_mg_shell_cells.argtypes = [vp]
     L.mf3d_mg_set_density_device.restype = ctypes.c_int
     L.mf3d_mg_set_density_device.argtypes = [vp, vp, vp]
     L.mf3d_mg_heaviside.restype = ctypes.c_int

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -31,7 +32,9 @@
 #include <vector>
 
 #include "../../include/mf3d_mg.h"
+#include "mg_coarse.hpp"
 #include "mg_common.hpp"
+#include "mg_ghostmap.hpp"
 #include "mg_outer.hpp"
 #include "mg_setup.hpp"
 #include "mg_sweep.hpp"
@@ -165,6 +168,26 @@ struct mf3d_mg {
     // ADI smoother: a(m) scratch of the per-pass kernels (level-0 size,
     // shared by the levels whose blocks do not fit in LDS)
     void *d_adi = nullptr;
+    // FP64 direct solve of the coarsest level: cell table, the inverse
+    // operator (nc x nc, rebuilt with the coefficients), a pivot-column
+    // scratch; cdirect set at build time
+    bool cdirect = false;
+    int nc = 0;
+    // FP64 sweeps of the mixed form's coarsest level (c64): its potential
+    // (two buffers) and RHS in FP64
+    bool c64 = false;
+    double *c64_phi[2] = {nullptr, nullptr}, *c64_rhs = nullptr;
+    CCell *d_ccell = nullptr;
+    double *d_cinv = nullptr, *d_ccol = nullptr;
+    // the zero-start sweep of the march levels runs k_rbz (the RHS alone);
+    // MF3D_ZERO_MARCH=0 keeps k_rbs from a zero potential (bitwise the same)
+    bool zmarch = true;
+    // the velocity form's fields on the caller's ghost shell: G layers
+    // (every block's `ghost`), per-block shell geometry, total length
+    int G = 1;
+    std::vector<BlockLevel> hw;
+    BlockLevel *d_vw = nullptr;
+    long long cells_w = 0;
     // allocation-free solve (mf3d_mg_solve_device_noalloc): every device
     // buffer is carved from this device's `scratch` array instead of
     // hipMalloc; no graphs, no tuner, no staging, the caller's stream
@@ -235,6 +258,12 @@ std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (c.precision != MF3D_FP64 && c.precision != MF3D_MIXED) return "unknown precision";
     if (c.coarse_op != MF3D_COARSE_REFERENCE && c.coarse_op != MF3D_COARSE_FACE) return "unknown coarse_op";
     if (c.krylov != MF3D_OUTER_DEFECT && c.krylov != MF3D_OUTER_GCR) return "unknown outer iteration (krylov)";
+    if (c.coarse_solve < MF3D_COARSEST_DEFAULT || c.coarse_solve > MF3D_COARSEST_DIRECT)
+        return "unknown coarsest-level solve (coarse_solve)";
+    if (c.pl_ex < 0) return "negative pl_ex";
+    if (c.differencing == 2 && c.variable_coef)
+        return "differencing = 2 with a variable coefficient: the reference runs its level set with WENO "
+               "differencing only (initial.for:133-135)";
     const int div = 1 << (c.levels - 1);
     for (int i = 0; i < c.nblocks; ++i) {
         for (int d = 0; d < 3; ++d) {
@@ -438,6 +467,21 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (c.smoother == 2) tot += gh0 * w;                 // ADI a(m) scratch (blocks too large for LDS)
 
     tot += std::max(gh0, in0) * 8;  // I/O staging
+    if (c.levels >= 2 && c.precision == MF3D_MIXED) {  // FP64 coarsest potential (two) and RHS
+        long long ghc = 0;
+        for (int i = 0; i < c.nblocks; ++i)
+            ghc += (long long)(dim_at(b[i], 0, c.levels - 1) + 2) * (dim_at(b[i], 1, c.levels - 1) + 2) *
+                   (dim_at(b[i], 2, c.levels - 1) + 2);
+        tot += 3 * 8 * ghc;
+    }
+    const bool direct = c.levels >= 2 && c.coarse_solve == MF3D_COARSEST_DIRECT;
+    if (direct) {  // the coarsest level's inverse operator
+        long long nc = 0;
+        for (int i = 0; i < c.nblocks; ++i)
+            nc += (long long)dim_at(b[i], 0, c.levels - 1) * dim_at(b[i], 1, c.levels - 1) *
+                  dim_at(b[i], 2, c.levels - 1);
+        if (nc <= MF3D_COARSEST_MAX_CELLS) tot += nc * nc * 8 + nc * 64;
+    }
     return tot;
 }
 
@@ -491,6 +535,9 @@ void drop_timed_graphs(mf3d_mg *mg) {
 // ---------------------------------------------------------------- launches
 inline const int *done_ptr(mf3d_mg *mg) { return &mg->st->done; }
 
+template <typename TD, typename TS>
+void convert(mf3d_mg *mg, TD *dst, const TS *src, long long n);
+
 template <typename T>
 void fill(mf3d_mg *mg, T *a, long long n, const int *done = nullptr) {
     if (n <= 0) return;
@@ -520,6 +567,17 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
         hipLaunchKernelGGL((k_rb3d<T, VAR, ZERO, RCP>), dim3(L.n_sw), dim3(R3_NT), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
                            L.t_sw, done);
+        return;
+    }
+    if (ZERO && mg->zmarch) {  // from zero: the march over the RHS alone
+        if (L.sw_ty == 8)
+            hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pout,
+                               (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
+                               L.t_sw, done);
+        else
+            hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pout,
+                               (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
+                               L.t_sw, done);
         return;
     }
     if (L.sw_ty == 8)
@@ -636,6 +694,55 @@ void prolong_to(mf3d_mg *mg, int lf, const int *done) {
     F.cur = 1 - F.cur;
 }
 
+// The reference's sweeps of the coarsest level in FP64 for the mixed form
+// (per-level path): widen the level's RHS, run the sweeps on an FP64 copy
+// of the level (same tiles, ghosts by the sweeps' own images or the
+// gather), narrow the potential, ghosts included.
+template <bool VAR>
+void coarse_smooth64(mf3d_mg *mg, int sweeps, const int *done) {
+    Level &C = mg->lv[mg->cfg.levels - 1];
+    Level S = C;  // the FP64 twin: same tables and tiles
+    S.phi[0] = mg->c64_phi[0];
+    S.phi[1] = mg->c64_phi[1];
+    S.rhs = mg->c64_rhs;
+    S.cx = C.dcx;
+    S.cy = C.dcy;
+    S.cz = C.dcz;
+    S.cur = 0;
+    convert<double, float>(mg, mg->c64_rhs, (const float *)C.rhs, C.cells_gh);
+    if (sweeps == 0) fill<double>(mg, (double *)S.phi[0], S.cells_gh, done);
+    for (int s = 0; s < sweeps; ++s) {
+        if (s == 0) sweep_launch<double, VAR, true>(mg, S, done);
+        else sweep_launch<double, VAR, false>(mg, S, done);
+        S.cur = 1 - S.cur;
+        if (S.n_full && !S.own_ghosts) {
+            const int gx = std::max(1, std::min(64, (S.max_full + 255) / 256));
+            hipLaunchKernelGGL(k_gather<double>, dim3(gx, S.n_full), dim3(256), 0, mg->stream,
+                               (double *)S.phi[S.cur], S.db, S.r_full, done);
+        }
+    }
+    convert<float, double>(mg, (float *)C.phi[1 - C.cur], (const double *)S.phi[S.cur], C.cells_gh);
+    C.cur = 1 - C.cur;
+}
+
+// The coarsest level's FP64 direct solve: phi = Minv * rhs on every block's
+// interior, then the ghost fill from the level's region table (lone blocks
+// included, whose sweeps otherwise store their own ghost images).
+template <typename T>
+void coarse_direct(mf3d_mg *mg, const int *done) {
+    Level &C = mg->lv[mg->cfg.levels - 1];
+    T *out = (T *)C.phi[1 - C.cur];
+    const unsigned gb = (unsigned)std::min(512, (mg->nc + 3) / 4);
+    hipLaunchKernelGGL(k_coarse_solve<T>, dim3(gb), dim3(256), sizeof(double) * (size_t)mg->nc, mg->stream,
+                       (const double *)mg->d_cinv, mg->nc, (const CCell *)mg->d_ccell, C.db, (const T *)C.rhs, out,
+                       done);
+    C.cur = 1 - C.cur;
+    if (C.n_full) {
+        const int gx = std::max(1, std::min(64, (C.max_full + 255) / 256));
+        hipLaunchKernelGGL(k_gather<T>, dim3(gx, C.n_full), dim3(256), 0, mg->stream, out, C.db, C.r_full, done);
+    }
+}
+
 // Levels lt..L-1 in one launch (k_tail); level lt's RHS is in place and its
 // potential starts from zero (or from phi_in when given).
 template <typename T, bool VAR>
@@ -650,7 +757,9 @@ void tail(mf3d_mg *mg, bool from_zero, const int *done) {
     }
     hipLaunchKernelGGL((k_tail<T, VAR>), dim3(mg->cfg.nblocks), dim3(TAIL_NT), mg->tail_lds, mg->stream, mg->d_tail,
                        mg->cfg.nblocks, L - lt, (const T *)V.rhs, from_zero ? nullptr : (const T *)V.phi[V.cur],
-                       (T *)V.phi[1 - V.cur], co, mg->cfg.nu_pre, mg->cfg.nu_post, done);
+                       (T *)V.phi[1 - V.cur], co, mg->cfg.nu_pre, mg->cfg.nu_post,
+                       mg->cdirect ? (const double *)mg->d_cinv : nullptr, mg->nc,
+                       TailC64<T>{mg->c64 ? 1 : 0, mg->lv[L - 1].dcx, mg->lv[L - 1].dcy, mg->lv[L - 1].dcz}, done);
     V.cur = 1 - V.cur;
 }
 
@@ -666,7 +775,10 @@ void vcycle_dev(mf3d_mg *mg, bool zero_start, const int *done) {
         const int ldown = std::min(lt, L);
         for (int l = 1; l < ldown; ++l) {
             restrict_to<T, VAR>(mg, l, done);
-            relax<T, VAR>(mg, l, mg->cfg.nu_pre + (l == L - 1 ? mg->cfg.nu_post : 0), true, done);
+            if (l == L - 1 && mg->cdirect) coarse_direct<T>(mg, done);
+            else if (l == L - 1 && mg->c64 && sizeof(T) == 4)
+                coarse_smooth64<VAR>(mg, mg->cfg.nu_pre + mg->cfg.nu_post, done);
+            else relax<T, VAR>(mg, l, mg->cfg.nu_pre + (l == L - 1 ? mg->cfg.nu_post : 0), true, done);
         }
         if (lt < L) {
             restrict_to<T, VAR>(mg, lt, done);
@@ -943,6 +1055,7 @@ int choose_tail(mf3d_mg *mg) {
     for (const mf3d_block_desc &B : mg->blk)
         for (int f = 0; f < 6; ++f)
             if (B.nbr[f] >= 0 || B.per[f] >= 0) return 0;
+    if (mg->cdirect && nb > 1) return 0;  // the tail's direct solve covers one block
     const size_t w = inner_bytes(mg->cfg);
     for (int lt = 0; lt < L; ++lt) {
         size_t need = 0;
@@ -951,6 +1064,16 @@ int choose_tail(mf3d_mg *mg) {
             for (int l = lt; l < L; ++l)
                 s += 2 * w * (size_t)(dim_at(B, 0, l) + 2) * (dim_at(B, 1, l) + 2) * (dim_at(B, 2, l) + 2);
             need = std::max(need, s);
+        }
+        // the direct solve's FP64 staging of the coarsest RHS, or the FP64
+        // potential and RHS of its sweeps, follows the level arrays
+        if (mg->cdirect) need = (need + 7) / 8 * 8 + sizeof(double) * (size_t)mg->nc;
+        else if (mg->c64) {
+            size_t ghc = 0;
+            for (const mf3d_block_desc &B : mg->blk)
+                ghc = std::max(ghc, (size_t)(dim_at(B, 0, L - 1) + 2) * (dim_at(B, 1, L - 1) + 2) *
+                                        (dim_at(B, 2, L - 1) + 2));
+            need = (need + 7) / 8 * 8 + 2 * sizeof(double) * ghc;
         }
         if (need <= kTailLdsMax) {
             mg->lt = lt;
@@ -977,6 +1100,131 @@ int choose_tail(mf3d_mg *mg) {
 
 template <typename T, bool VAR>
 int autotune_sweeps(mf3d_mg *mg);
+void shell_geometry(mf3d_mg *mg);
+
+// Whether the coarsest level is solved directly (MF3D_COARSEST_*), and
+// its cell count.  Called once the level tables exist, before the tail.
+int decide_coarse(mf3d_mg *mg) {
+    const mf3d_mg_cfg &c = mg->cfg;
+    const int L = c.levels;
+    mg->cdirect = false;
+    mg->nc = 0;
+    if (L < 2) return 0;  // one level: the reference's smoothing is the whole cycle
+    // the reference's sweeps in FP64 on the coarsest level of the mixed
+    // form (red-black smoother; the in-place smoothers keep the inner
+    // precision there)
+    mg->c64 = c.precision == MF3D_MIXED && c.coarse_solve == MF3D_COARSEST_DEFAULT && c.smoother == 3;
+    if (c.coarse_solve != MF3D_COARSEST_DIRECT) return 0;
+    long long nc = 0;
+    for (const BlockLevel &g : mg->lv[L - 1].hb) nc += (long long)g.n0 * g.n1 * g.n2;
+    if (nc > MF3D_COARSEST_MAX_CELLS) {
+        mg->err = "the coarsest level has " + std::to_string(nc) + " cells, more than the " +
+                  std::to_string(MF3D_COARSEST_MAX_CELLS) + " of its direct solve: use more levels";
+        return MF3D_EINVAL;
+    }
+    mg->cdirect = true;
+    mg->nc = (int)nc;
+    return 0;
+}
+
+// The coarsest level's cell table: every cell's block and index, its six
+// neighbours as cells (a ghost neighbour resolved through the level's
+// region table to the cell the exchange copies into it), and one pinned
+// cell per group of blocks connected through neighbours or wraps (the
+// operator's constants); then the matrix buffers.
+int coarse_setup(mf3d_mg *mg) {
+    const int L = mg->cfg.levels, nb = mg->cfg.nblocks;
+    const Level &C = mg->lv[L - 1];
+    int mx = 0;
+    const std::vector<Region> regs = build_regions(mg->blk, L - 1, &mx);
+    std::vector<int> base(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) base[b + 1] = base[b] + C.hb[b].n0 * C.hb[b].n1 * C.hb[b].n2;
+    auto idx = [&](int b, int i, int j, int k) {
+        const BlockLevel &g = C.hb[b];
+        return base[b] + (i - 1) + g.n0 * ((j - 1) + g.n1 * (k - 1));
+    };
+    auto resolve = [&](int b, int i, int j, int k) -> int {
+        const BlockLevel &g = C.hb[b];
+        if (i >= 1 && i <= g.n0 && j >= 1 && j <= g.n1 && k >= 1 && k <= g.n2) return idx(b, i, j, k);
+        const int v[3] = {i, j, k};
+        for (const Region &r : regs) {
+            if (r.dst_b != b) continue;
+            bool in = true;
+            for (int a = 0; a < 3; ++a) in = in && v[a] >= r.lo[a] && v[a] < r.lo[a] + r.cnt[a];
+            if (!in) continue;
+            int s[3];
+            for (int a = 0; a < 3; ++a) s[a] = r.map[a] < 0 ? v[a] : r.map[a];
+            return idx(r.src_b, s[0], s[1], s[2]);
+        }
+        return -1;
+    };
+    const int nc = mg->nc;
+    std::vector<CCell> cells(nc);
+    std::vector<int> parent(nc);
+    for (int q = 0; q < nc; ++q) parent[q] = q;
+    std::function<int(int)> root = [&](int q) { return parent[q] == q ? q : parent[q] = root(parent[q]); };
+    for (int b = 0; b < nb; ++b) {
+        const BlockLevel &g = C.hb[b];
+        for (int k = 1; k <= g.n2; ++k)
+            for (int j = 1; j <= g.n1; ++j)
+                for (int i = 1; i <= g.n0; ++i) {
+                    CCell &e = cells[idx(b, i, j, k)];
+                    e.b = b; e.i = i; e.j = j; e.k = k; e.pin = 0;
+                    const int nbq[6] = {resolve(b, i - 1, j, k), resolve(b, i + 1, j, k), resolve(b, i, j - 1, k),
+                                        resolve(b, i, j + 1, k), resolve(b, i, j, k - 1), resolve(b, i, j, k + 1)};
+                    for (int d = 0; d < 6; ++d) {
+                        if (nbq[d] < 0) {
+                            mg->err = "coarsest level: a ghost with no region (internal)";
+                            return MF3D_EDEVICE;
+                        }
+                        e.nb[d] = nbq[d];
+                        const int ra = root(idx(b, i, j, k)), rb = root(nbq[d]);
+                        if (ra != rb) parent[ra] = rb;
+                    }
+                }
+    }
+    std::vector<int> last(nc, -1);  // the last cell of each connected group is pinned
+    for (int q = 0; q < nc; ++q) last[root(q)] = q;
+    for (int q = 0; q < nc; ++q)
+        if (last[q] >= 0) cells[last[q]].pin = 1;
+    int rc;
+    if ((rc = upload(mg, &mg->d_ccell, cells))) return rc;
+    if ((rc = dev_alloc(mg, (void **)&mg->d_cinv, sizeof(double) * (size_t)nc * nc)) ||
+        (rc = dev_alloc(mg, (void **)&mg->d_ccol, sizeof(double) * (size_t)nc)))
+        return rc;
+    return 0;
+}
+
+// Assemble the coarsest operator from the current coefficients (FP64: the
+// constant 1/h^2 or the FP64 face coefficients) and invert it in place.
+template <bool VAR>
+int coarse_factor(mf3d_mg *mg) {
+    if (!mg->cdirect) return 0;
+    const Level &C = mg->lv[mg->cfg.levels - 1];
+    const int nc = mg->nc;
+    fill<double>(mg, mg->d_cinv, (long long)nc * nc);
+    hipLaunchKernelGGL(k_coarse_assemble<VAR>, dim3((nc + 255) / 256), dim3(256), 0, mg->stream, mg->d_cinv, nc,
+                       (const CCell *)mg->d_ccell, C.db, (const double *)C.dcx, (const double *)C.dcy,
+                       (const double *)C.dcz);
+    if (nc <= 512) {
+        hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, mg->stream, mg->d_cinv, nc, mg->d_ccol);
+    } else {
+        const unsigned g1 = (unsigned)((nc + 255) / 256);
+        for (int k = 0; k < nc; ++k) {
+            hipLaunchKernelGGL(k_gj_col, dim3(g1), dim3(256), 0, mg->stream, (const double *)mg->d_cinv, nc, k,
+                               mg->d_ccol);
+            hipLaunchKernelGGL(k_gj_row, dim3(g1), dim3(256), 0, mg->stream, mg->d_cinv, nc, k,
+                               (const double *)mg->d_ccol);
+            hipLaunchKernelGGL(k_gj_update, dim3(g1, nc), dim3(256), 0, mg->stream, mg->d_cinv, nc, k,
+                               (const double *)mg->d_ccol);
+        }
+    }
+    hipLaunchKernelGGL(k_coarse_unpin, dim3((nc + 255) / 256, nc), dim3(256), 0, mg->stream, mg->d_cinv, nc,
+                       (const CCell *)mg->d_ccell);
+    HIPCHK(hipStreamSynchronize(mg->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
 
 // ADI smoother set-up: per level, the whole-sweep LDS kernel where the
 // level's largest block (its ghosted potential and the per-diagonal
@@ -1048,6 +1296,10 @@ int adi_coef_build(mf3d_mg *mg) {
 
 int build(mf3d_mg *mg) {
     const mf3d_mg_cfg &c = mg->cfg;
+    {
+        const char *e = std::getenv("MF3D_ZERO_MARCH");
+        mg->zmarch = !(e && e[0] == '0');
+    }
     const int L = c.levels, nb = c.nblocks;
     const bool mixed = c.precision == MF3D_MIXED;
     const size_t w = mixed ? 4 : 8;
@@ -1148,7 +1400,15 @@ int build(mf3d_mg *mg) {
         mg->n_out = (int)tout.size();
         if ((rc = upload(mg, &mg->t_out, tout))) return rc;
     }
-    if ((rc = choose_tail(mg))) return rc;
+    if ((rc = decide_coarse(mg)) || (rc = choose_tail(mg))) return rc;
+    if (mg->cdirect && (rc = coarse_setup(mg))) return rc;
+    if (mg->c64) {
+        const long long ghc = mg->lv[L - 1].cells_gh;
+        if ((rc = dev_alloc(mg, (void **)&mg->c64_phi[0], 8 * ghc)) ||
+            (rc = dev_alloc(mg, (void **)&mg->c64_phi[1], 8 * ghc)) ||
+            (rc = dev_alloc(mg, (void **)&mg->c64_rhs, 8 * ghc)))
+            return rc;
+    }
     if ((rc = dev_alloc(mg, (void **)&mg->pbuf[0], L0.cells_gh * 8)) ||
         (rc = dev_alloc(mg, (void **)&mg->pbuf[1], L0.cells_gh * 8)) ||
         (rc = dev_alloc(mg, (void **)&mg->f_g, L0.cells_gh * 8)))
@@ -1165,8 +1425,9 @@ int build(mf3d_mg *mg) {
         (rc = dev_alloc(mg, (void **)&mg->d_hist, sizeof(double) * (c.maxcy + 1))) ||
         (rc = dev_alloc(mg, (void **)&mg->d_rmax, sizeof(double) * (c.maxcy + 1))))
         return rc;
+    shell_geometry(mg);
     if (!mg->arena) {  // host-array entries' staging (the allocation-free solve has none)
-        mg->io_len = std::max(L0.cells_gh, mg->cells);
+        mg->io_len = std::max({L0.cells_gh, mg->cells, mg->cells_w});
         if ((rc = dev_alloc(mg, (void **)&mg->d_io, sizeof(double) * mg->io_len))) return rc;
         HIPCHK(hipHostMalloc((void **)&mg->h_io, sizeof(double) * mg->io_len, hipHostMallocDefault));
     }
@@ -1190,6 +1451,8 @@ int build(mf3d_mg *mg) {
     hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, mg->stream, mg->st, mg->cfg.eps);
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
+    // constant coefficients are final now; variable ones at each coeff
+    if (!c.variable_coef && (rc = coarse_factor<false>(mg))) return rc;
     if (c.smoother == 2) {
         if ((rc = adi_setup(mg))) return rc;
         bool global = false;  // a level runs the per-pass kernels: their a(m) scratch
@@ -1394,6 +1657,7 @@ int build_lsm(mf3d_mg *mg) {
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
     mg->density_set = true;
+    if (int rc = coarse_factor<true>(mg)) return rc;  // the coarsest inverse follows the operator
     if (mg->cfg.smoother == 2) {  // the ADI Thomas coefficients follow the operator
         const int rc = mg->cfg.precision == MF3D_MIXED ? adi_coef_build<float, true>(mg)
                                                        : adi_coef_build<double, true>(mg);
@@ -1468,289 +1732,27 @@ void drop_graphs(mf3d_mg *mg) {
 
 // ----------------------------------------------------- velocity form (host)
 //
-// exchange(op) and pbound only copy values, so one call of them is a fixed
-// map: every ghost entry it writes ends up holding the value some entry
-// (interior or ghost) held before the call.  The map is found once, by
-// running the reference's copy sequence on origins instead of values, and
-// then applied on the device as a gather.  The sequence, for one ghost
-// layer on one rank with equal resolution:
-//   exchange_bc (exchange_bc.for:14-841), when any face is periodic: for
-//     every periodic face without a neighbour, in block order and face
-//     order W, E, S, N, B, T, a block alone along that axis copies its own
-//     opposite plane, otherwise it hands its boundary plane to the
-//     periodic partner; whole ghosted planes; the hand-over loop runs
-//     twice (:16-330, :332-640) and the partners unpack last (:642-838);
-//   exchange_smlvl (exchange.for:342-2499): every block packs before any
-//     block unpacks; each block then unpacks its 8 corners, its 12 edges
-//     and its 6 faces, in that order (:2118-2484);
-//   pbound (press.for:77-137, pressure only): each physical non-periodic
-//     face, W, E, S, N, B, T, copies the adjacent interior plane over its
-//     whole ghosted plane.
-
-// Ghost entries of one field, per block: the arena offset of the entry
-// whose original value each ghost holds.  Slots: the x ghost planes
-// (whole), then the y ghost planes (interior i), then the z ghost planes
-// (interior i, j).  The few interior entries a sequence rewrites
-// (bound_lsm's north rows) are kept by arena offset in `inner`.
-struct GhostOrigins {
-    const std::vector<BlockLevel> &hb;
-    std::vector<std::vector<uint64_t>> o;
-    std::map<uint64_t, uint64_t> inner;
-
-    explicit GhostOrigins(const std::vector<BlockLevel> &h) : hb(h), o(h.size()) {
-        for (size_t b = 0; b < h.size(); ++b) {
-            const BlockLevel &g = h[b];
-            o[b].resize(2ull * (g.n1 + 2) * (g.n2 + 2) + 2ull * g.n0 * (g.n2 + 2) + 2ull * g.n0 * g.n1);
-            for_ghosts((int)b, [&](int i, int j, int k) { o[b][slot((int)b, i, j, k)] = self(g, i, j, k); });
-        }
-    }
-    static uint64_t self(const BlockLevel &g, int i, int j, int k) {
-        return (uint64_t)(g.off + i + g.sy * j + g.sz * k);
-    }
-    long long slot(int b, int i, int j, int k) const {
-        const BlockLevel &g = hb[b];
-        const long long X = (long long)(g.n1 + 2) * (g.n2 + 2), Y = (long long)g.n0 * (g.n2 + 2);
-        if (i == 0 || i == g.n0 + 1) return (i == 0 ? 0 : X) + j + (long long)(g.n1 + 2) * k;
-        if (j == 0 || j == g.n1 + 1) return 2 * X + (j == 0 ? 0 : Y) + (i - 1) + (long long)g.n0 * k;
-        if (k == 0 || k == g.n2 + 1)
-            return 2 * X + 2 * Y + (k == 0 ? 0 : (long long)g.n0 * g.n1) + (i - 1) + (long long)g.n0 * (j - 1);
-        return -1;
-    }
-    uint64_t get(int b, int i, int j, int k) const {
-        const long long s = slot(b, i, j, k);
-        if (s >= 0) return o[b][s];
-        const uint64_t me = self(hb[b], i, j, k);
-        const auto it = inner.find(me);
-        return it == inner.end() ? me : it->second;
-    }
-    void set(int b, int i, int j, int k, uint64_t v) {
-        const long long s = slot(b, i, j, k);
-        if (s >= 0) o[b][s] = v;
-        else inner[self(hb[b], i, j, k)] = v;
-    }
-    template <typename Fn>
-    void for_ghosts(int b, Fn fn) const {
-        const BlockLevel &g = hb[b];
-        for (int k = 0; k <= g.n2 + 1; ++k)
-            for (int j = 0; j <= g.n1 + 1; ++j) {
-                const bool edge = k == 0 || k == g.n2 + 1 || j == 0 || j == g.n1 + 1;
-                if (edge)
-                    for (int i = 0; i <= g.n0 + 1; ++i) fn(i, j, k);
-                else {
-                    fn(0, j, k);
-                    fn(g.n0 + 1, j, k);
-                }
-            }
-    }
-};
-
-inline int ext_of(const BlockLevel &g, int a) { return a == 0 ? g.n0 : (a == 1 ? g.n1 : g.n2); }
-
-// (i, j, k) with coordinate `idx` on axis a and (p, q) on the two other
-// axes in increasing order.
-inline void plane_ijk(int a, int idx, int p, int q, int *ijk) {
-    ijk[a] = idx;
-    ijk[a == 0 ? 1 : 0] = p;
-    ijk[a == 2 ? 1 : 2] = q;
-}
-
-// Last face index of component comp on axis a (n, or n-1 before a physical
-// non-periodic wall without a neighbour); cell fields (comp < 0): n.
-inline int face_end(const mf3d_block_desc &B, int a, int comp) {
-    const int n = B.n[a];
-    return (comp == a && B.nbr[2 * a + 1] < 0 && B.bc[2 * a + 1] != 5) ? n - 1 : n;
-}
-
-void sym_exchange_bc(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S, int comp) {
-    std::map<std::pair<int, int>, std::vector<uint64_t>> bufs;
-    const int nb = (int)blk.size();
-    auto plane_each = [&](int b, int a, auto fn) {
-        const BlockLevel &g = S.hb[b];
-        const int e1 = ext_of(g, a == 0 ? 1 : 0) + 2, e2 = ext_of(g, a == 2 ? 1 : 2) + 2;
-        for (int q = 0; q < e2; ++q)
-            for (int p = 0; p < e1; ++p) fn(p, q);
-    };
-    for (int pass = 0; pass < 2; ++pass)
-        for (int b = 0; b < nb; ++b)
-            for (int a = 0; a < 3; ++a)
-                for (int side = 0; side < 2; ++side) {
-                    const int f = 2 * a + side;
-                    if (!(blk[b].bc[f] == 5 && blk[b].nbr[f] < 0)) continue;
-                    const int ie = face_end(blk[b], a, comp);
-                    if (blk[b].nbr[2 * a + 1 - side] < 0) {  // alone along a: own opposite plane
-                        const int dst = side == 0 ? 0 : ie + 1, src = side == 0 ? ie : 1;
-                        plane_each(b, a, [&](int p, int q) {
-                            int d[3], s[3];
-                            plane_ijk(a, dst, p, q, d);
-                            plane_ijk(a, src, p, q, s);
-                            S.set(b, d[0], d[1], d[2], S.get(b, s[0], s[1], s[2]));
-                        });
-                    } else {
-                        std::vector<uint64_t> &buf = bufs[{blk[b].per[f], 2 * a + 1 - side}];
-                        buf.clear();
-                        const int src = side == 0 ? 1 : ie;
-                        plane_each(b, a, [&](int p, int q) {
-                            int s[3];
-                            plane_ijk(a, src, p, q, s);
-                            buf.push_back(S.get(b, s[0], s[1], s[2]));
-                        });
-                    }
-                }
-    for (int b = 0; b < nb; ++b)
-        for (int a = 0; a < 3; ++a)
-            for (int side = 0; side < 2; ++side) {
-                const int f = 2 * a + side;
-                if (!(blk[b].bc[f] == 5 && blk[b].nbr[f] < 0 && blk[b].nbr[2 * a + 1 - side] >= 0)) continue;
-                const int dst = side == 0 ? 0 : face_end(blk[b], a, comp) + 1;
-                const auto it = bufs.find({b, f});
-                if (it == bufs.end()) continue;  // no partner handed a plane over
-                const std::vector<uint64_t> &buf = it->second;
-                size_t t = 0;
-                plane_each(b, a, [&](int p, int q) {
-                    int d[3];
-                    plane_ijk(a, dst, p, q, d);
-                    S.set(b, d[0], d[1], d[2], buf[t++]);
-                });
-            }
-}
-
-// Block in direction t (components -1, 0, +1) through chained face
-// neighbours (no periodic wrap), or -1: the edge and corner neighbours that
-// alloc_dom.for:617-1163 finds by matching block coordinates.  Every order
-// of the chain must agree (a conforming block grid).
-int diag_block(const std::vector<mf3d_block_desc> &blk, int b, const int *t, bool *ambiguous) {
-    int res = -2;
-    const int orders[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-    for (const auto &ord : orders) {
-        int c = b;
-        for (int m = 0; m < 3 && c >= 0; ++m) {
-            const int a = ord[m];
-            if (t[a]) c = blk[c].nbr[2 * a + (t[a] > 0 ? 1 : 0)];
-        }
-        if (res == -2) res = c;
-        else if (res != c) *ambiguous = true;
-    }
-    return res;
-}
-
-// corprev1..4, cornext1..4, edgprev1..6, edgnext1..6 (exchange.for:2118-2389)
-constexpr int kDiagDirs[20][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1}, {1, 1, 1},
-                                  {1, -1, 1},   {-1, -1, 1}, {-1, 1, 1}, {-1, 0, -1}, {0, 1, -1},
-                                  {1, 0, -1},   {0, -1, -1}, {-1, -1, 0}, {-1, 1, 0}, {1, 0, 1},
-                                  {0, -1, 1},   {-1, 0, 1},  {0, 1, 1},  {1, 1, 0},   {1, -1, 0}};
-
-bool sym_exchange_smlvl(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
-    const GhostOrigins pre = S;
-    bool ambiguous = false;
-    const int nb = (int)blk.size();
-    for (int d = 0; d < nb; ++d) {
-        const BlockLevel &gd = S.hb[d];
-        const int nd[3] = {gd.n0, gd.n1, gd.n2};
-        // corners and edges: along an axis where the sender lies below the
-        // receiver's index 1-q takes the sender's n+1-q, above its n+q the
-        // sender's q (q = 0, 1); written where some q = 1; edges run over
-        // the interior of their free axis
-        for (const auto &t : kDiagDirs) {
-            const int s = diag_block(blk, d, t, &ambiguous);
-            if (s < 0) continue;
-            const BlockLevel &gs = S.hb[s];
-            const int ns[3] = {gs.n0, gs.n1, gs.n2};
-            bool conform = true;
-            for (int a = 0; a < 3; ++a) conform = conform && (t[a] != 0 || ns[a] == nd[a]);
-            if (!conform) {  // an edge neighbour must span the same cells along its free axis
-                ambiguous = true;
-                continue;
-            }
-            int lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = t[a] ? 0 : 1;
-                hi[a] = t[a] ? 1 : nd[a];
-            }
-            for (int v2 = lo[2]; v2 <= hi[2]; ++v2)
-                for (int v1 = lo[1]; v1 <= hi[1]; ++v1)
-                    for (int v0 = lo[0]; v0 <= hi[0]; ++v0) {
-                        const int v[3] = {v0, v1, v2};
-                        int di[3], si[3];
-                        bool any = false;
-                        for (int a = 0; a < 3; ++a) {
-                            if (!t[a]) {
-                                di[a] = si[a] = v[a];
-                                continue;
-                            }
-                            const int q = v[a];
-                            any = any || q == 1;
-                            if (t[a] < 0) {
-                                di[a] = 1 - q;
-                                si[a] = ns[a] + 1 - q;
-                            } else {
-                                di[a] = nd[a] + q;
-                                si[a] = q;
-                            }
-                        }
-                        if (any) S.set(d, di[0], di[1], di[2], pre.get(s, si[0], si[1], si[2]));
-                    }
-        }
-        // faces: the interior rows of the ghost plane from the sender's
-        // boundary plane
-        for (int a = 0; a < 3; ++a)
-            for (int side = 0; side < 2; ++side) {
-                const int s = blk[d].nbr[2 * a + side];
-                if (s < 0) continue;
-                const int ns = ext_of(S.hb[s], a);
-                const int dst = side == 0 ? 0 : nd[a] + 1, src = side == 0 ? ns : 1;
-                const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
-                for (int q = 1; q <= nd[o2]; ++q)
-                    for (int p = 1; p <= nd[o1]; ++p) {
-                        int di[3], si[3];
-                        plane_ijk(a, dst, p, q, di);
-                        plane_ijk(a, src, p, q, si);
-                        S.set(d, di[0], di[1], di[2], pre.get(s, si[0], si[1], si[2]));
-                    }
-            }
-    }
-    return !ambiguous;
-}
-
-void sym_pbound(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
-    for (int b = 0; b < (int)blk.size(); ++b) {
-        const BlockLevel &g = S.hb[b];
-        for (int a = 0; a < 3; ++a)
-            for (int side = 0; side < 2; ++side) {
-                if (!is_neumann(blk[b], 2 * a + side)) continue;
-                const int n = ext_of(g, a);
-                const int dst = side == 0 ? 0 : n + 1, src = side == 0 ? 1 : n;
-                const int e1 = ext_of(g, a == 0 ? 1 : 0) + 2, e2 = ext_of(g, a == 2 ? 1 : 2) + 2;
-                for (int q = 0; q < e2; ++q)
-                    for (int p = 0; p < e1; ++p) {
-                        int d[3], s[3];
-                        plane_ijk(a, dst, p, q, d);
-                        plane_ijk(a, src, p, q, s);
-                        S.set(b, d[0], d[1], d[2], S.get(b, s[0], s[1], s[2]));
-                    }
-            }
-    }
-}
+// exchange(1..4), pbound and bound_lsm only copy values; their sequences
+// are run once on origins (mg_ghostmap.hpp) and applied as gather maps.
 
 // bound_lsm (bounds_lsm.for:30-97) with one ghost layer: on each physical
 // non-periodic face, in the order E, W, B, T, S, N, the whole ghosted
 // ghost plane takes the plane inside it; the north loop starts at ipl = 0
 // (bounds_lsm.for:91), so the last interior row first takes row n-1.
-void sym_bound_lsm(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
+void shell_bound_lsm(const std::vector<mf3d_block_desc> &blk, ShellOrigins &S) {
     constexpr int order[6] = {1, 0, 4, 5, 2, 3};
     for (int b = 0; b < (int)blk.size(); ++b) {
         const BlockLevel &g = S.hb[b];
         for (const int f : order) {
             if (!is_neumann(blk[b], f)) continue;
-            const int a = f / 2, n = ext_of(g, a);
-            const int e1 = ext_of(g, a == 0 ? 1 : 0) + 2, e2 = ext_of(g, a == 2 ? 1 : 2) + 2;
+            const int a = f / 2, n = shell_ext(g, a);
             auto copy = [&](int dst, int src) {
-                for (int q = 0; q < e2; ++q)
-                    for (int p = 0; p < e1; ++p) {
-                        int d[3], s[3];
-                        plane_ijk(a, dst, p, q, d);
-                        plane_ijk(a, src, p, q, s);
-                        S.set(b, d[0], d[1], d[2], S.get(b, s[0], s[1], s[2]));
-                    }
+                shell_plane(g, S.G, a, [&](int p, int q) {
+                    int d[3], s2[3];
+                    shell_ijk(a, dst, p, q, d);
+                    shell_ijk(a, src, p, q, s2);
+                    S.set(b, d[0], d[1], d[2], S.get(b, s2[0], s2[1], s2[2]));
+                });
             };
             if (f % 2 == 0) copy(0, 1);
             else {
@@ -1761,32 +1763,51 @@ void sym_bound_lsm(const std::vector<mf3d_block_desc> &blk, GhostOrigins &S) {
     }
 }
 
-// Map entries {destination, source} of the entries the sequence changed.
-void map_entries(const GhostOrigins &S, std::vector<uint2> &out) {
-    for (int b = 0; b < (int)S.hb.size(); ++b) {
-        const BlockLevel &g = S.hb[b];
-        S.for_ghosts(b, [&](int i, int j, int k) {
-            const uint64_t self = GhostOrigins::self(g, i, j, k), org = S.get(b, i, j, k);
-            if (org != self) out.push_back(make_uint2((uint32_t)self, (uint32_t)org));
-        });
+// The velocity form's shell geometry: every block's array with G = ghost
+// layers, blocks one after another; off addresses coordinate (0,0,0).
+void shell_geometry(mf3d_mg *mg) {
+    const Level &L0 = mg->lv[0];
+    mg->G = std::max(1, mg->blk[0].ghost);
+    const int G = mg->G;
+    mg->hw = L0.hb;
+    long long base = 0;
+    for (size_t b = 0; b < mg->hw.size(); ++b) {
+        BlockLevel &g = mg->hw[b];
+        g.sy = g.n0 + 2 * G;
+        g.sz = g.sy * (g.n1 + 2 * G);
+        g.off = base + (G - 1) * (1 + g.sy + g.sz);
+        base += g.sz * (g.n2 + 2 * G);
     }
-    for (const auto &e : S.inner)
-        if (e.first != e.second) out.push_back(make_uint2((uint32_t)e.first, (uint32_t)e.second));
+    mg->cells_w = base;
 }
 
 int build_vel(mf3d_mg *mg) {
     if (mg->vel_ready) return 0;
     const Level &L0 = mg->lv[0];
-    const int nb = mg->cfg.nblocks;
-    if (L0.cells_gh >= (1LL << 32)) {
-        mg->err = "velocity form: a level-0 field holds 2^32 entries or more";
+    const int nb = mg->cfg.nblocks, G = mg->G;
+    for (const mf3d_block_desc &B : mg->blk)
+        if (std::max(1, B.ghost) != G) {
+            mg->err = "velocity form: every block needs the same ghost width";
+            return MF3D_EINVAL;
+        }
+    if (G < mg->cfg.pl_ex + 1) {
+        mg->err = "velocity form: pl_ex = " + std::to_string(mg->cfg.pl_ex) + " needs ghost >= " +
+                  std::to_string(mg->cfg.pl_ex + 1);
+        return MF3D_EINVAL;
+    }
+    if (mg->cfg.differencing == 2 && G < 2) {
+        mg->err = "velocity form: the 4th-order calmas (differencing = 2) reads two ghost layers: ghost >= 2";
+        return MF3D_EINVAL;
+    }
+    if (mg->cells_w >= (1LL << 32) || L0.cells_gh >= (1LL << 32)) {
+        mg->err = "velocity form: a field holds 2^32 entries or more";
         return MF3D_ENOTSUP;
     }
     std::vector<VelBlock> vb(nb);
     for (int b = 0; b < nb; ++b) {
         const mf3d_block_desc &B = mg->blk[b];
         for (int a = 0; a < 3; ++a) {
-            vb[b].ie[a] = face_end(B, a, a);
+            vb[b].ie[a] = shell_end(B, a, a);
             const int f = 2 * a;
             if (B.nbr[f] >= 0) vb[b].lo_src[a] = B.nbr[f];
             else if (B.bc[f] == 5) vb[b].lo_src[a] = B.nbr[f + 1] < 0 ? b : B.per[f];
@@ -1798,13 +1819,13 @@ int build_vel(mf3d_mg *mg) {
     });
     std::vector<uint2> ent;
     bool ok = true;
-    for (int comp = 0; comp < 4; ++comp) {  // u, v, w, then p (cell field)
-        GhostOrigins S(L0.hb);
-        const int c = comp < 3 ? comp : -1;
-        if (periodic) sym_exchange_bc(mg->blk, S, c);
-        ok = sym_exchange_smlvl(mg->blk, S) && ok;
-        if (comp == 3) sym_pbound(mg->blk, S);
-        map_entries(S, ent);
+    for (int comp = 0; comp < 4; ++comp) {  // u, v, w (pl_ex extra layers), then p (cell field, one layer)
+        ShellOrigins S(mg->hw, G, (size_t)mg->cells_w);
+        const int c = comp < 3 ? comp : -1, nly = comp < 3 ? mg->cfg.pl_ex : 0;
+        if (periodic) shell_exchange_bc(mg->blk, S, c, nly);
+        ok = shell_exchange_smlvl(mg->blk, S, c, nly) && ok;
+        if (comp == 3) shell_pbound(mg->blk, S);
+        shell_entries(S, ent);
         if (comp < 3) mg->vmap_c[comp] = (long long)ent.size();
     }
     if (!ok) {
@@ -1819,6 +1840,7 @@ int build_vel(mf3d_mg *mg) {
         if ((rc = upload(mg, &mg->t_vel, tv))) return rc;
     }
     if ((rc = upload(mg, &mg->d_vb, vb)) || (rc = upload(mg, &mg->d_vmap, ent)) ||
+        (rc = upload(mg, &mg->d_vw, mg->hw)) ||
         (rc = dev_alloc(mg, (void **)&mg->d_vstage, sizeof(double) * std::max<size_t>(ent.size(), 1))))
         return rc;
     mg->vel_ready = true;
@@ -1841,9 +1863,14 @@ template <typename T, bool VAR>
 void vel_step(mf3d_mg *mg, const VelFields &F, const int *done, bool update) {
     Level &L0 = mg->lv[0];
     if (update) {
-        const unsigned gb = (unsigned)std::min<long long>((L0.cells_gh + 255) / 256, 16384);
-        hipLaunchKernelGGL(k_vel_pupdate<T>, dim3(gb), dim3(256), 0, mg->stream, (double *)F.p,
-                           (const T *)L0.phi[L0.cur], L0.cells_gh, done);
+        if (mg->G == 1) {  // shell and level-0 layouts equal
+            const unsigned gb = (unsigned)std::min<long long>((L0.cells_gh + 255) / 256, 16384);
+            hipLaunchKernelGGL(k_vel_pupdate<T>, dim3(gb), dim3(256), 0, mg->stream, (double *)F.p,
+                               (const T *)L0.phi[L0.cur], L0.cells_gh, done);
+        } else {
+            hipLaunchKernelGGL(k_vel_pupdate_shell<T>, dim3(L0.n_row), dim3(kTileX, kTileY), 0, mg->stream,
+                               (double *)F.p, (const T *)L0.phi[L0.cur], L0.db, mg->d_vw, L0.t_row, done);
+        }
         const long long np = mg->vmap_p;
         map_launch(mg, mg->d_vmap + mg->vmap_c[2], MapArgs{{(double *)F.p, (double *)F.p, (double *)F.p}, {np, np, np}},
                    done);
@@ -1851,10 +1878,14 @@ void vel_step(mf3d_mg *mg, const VelFields &F, const int *done, bool update) {
     const bool tk = tk_on(mg, MF3D_K_OUTER, 0);
     if (tk) tk_mark(mg);
     hipLaunchKernelGGL((k_velmas<T, VAR, false, 8>), dim3(mg->vel_wg), dim3(SW_TX, 8), 0, mg->stream, F,
-                       (T *)L0.rhs, (double *)nullptr, mg->d_vsink, mg->st, L0.db, mg->d_vb, mg->t_vel, mg->n_vel,
-                       mg->d_vpart, done);
+                       (T *)L0.rhs, (double *)nullptr, mg->d_vsink, mg->st, L0.db, mg->d_vw, mg->d_vb, mg->t_vel,
+                       mg->n_vel, mg->d_vpart, done);
     if (tk) tk_mark(mg);
     map_launch(mg, mg->d_vmap, MapArgs{{F.u, F.v, F.w}, {mg->vmap_c[0], mg->vmap_c[1], mg->vmap_c[2]}}, done);
+    if (mg->cfg.differencing == 2)  // 4th order: on the exchanged faces; replaces the RHS and rmax above
+        hipLaunchKernelGGL((k_calmas4<T, false>), dim3(mg->vel_wg), dim3(kTileX, kTileY), 0, mg->stream, F,
+                           (T *)L0.rhs, (double *)nullptr, mg->st, L0.db, mg->d_vw, L0.t_in, L0.n_in, mg->d_vpart,
+                           done);
     hipLaunchKernelGGL(k_vel_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->d_vpart, mg->vel_wg, mg->st,
                        mg->d_rmax);
 }
@@ -1934,9 +1965,14 @@ int newsolv_dev(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, doubl
     mg->guess = std::max(k, 1);
     if (su) {  // the last calmas once more, its su in FP64 (faces rewritten unchanged)
         Level &L0 = mg->lv[0];
-        hipLaunchKernelGGL((k_velmas<double, VAR, true, 8>), dim3(mg->vel_wg), dim3(SW_TX, 8), 0, mg->stream, F,
-                           (double *)nullptr, su, mg->d_vsink, mg->st, L0.db, mg->d_vb, mg->t_vel, mg->n_vel,
-                           (double *)nullptr, (const int *)nullptr);
+        if (mg->cfg.differencing == 2)
+            hipLaunchKernelGGL((k_calmas4<double, true>), dim3(mg->vel_wg), dim3(kTileX, kTileY), 0, mg->stream,
+                               F, (double *)nullptr, su, mg->st, L0.db, mg->d_vw, L0.t_in, L0.n_in,
+                               (double *)nullptr, (const int *)nullptr);
+        else
+            hipLaunchKernelGGL((k_velmas<double, VAR, true, 8>), dim3(mg->vel_wg), dim3(SW_TX, 8), 0, mg->stream,
+                               F, (double *)nullptr, su, mg->d_vsink, mg->st, L0.db, mg->d_vw, mg->d_vb, mg->t_vel,
+                               mg->n_vel, (double *)nullptr, (const int *)nullptr);
     }
     std::vector<double> rm(k + 1);
     HIPCHK(hipMemcpyAsync(rm.data(), mg->d_rmax, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, mg->stream));
@@ -1982,15 +2018,15 @@ int build_ls(mf3d_mg *mg) {
     const bool periodic = std::any_of(mg->blk.begin(), mg->blk.end(), [](const mf3d_block_desc &B) {
         return std::any_of(B.bc, B.bc + 6, [](int c) { return c == 5; });
     });
-    GhostOrigins S(L0.hb);
-    sym_bound_lsm(mg->blk, S);
-    if (periodic) sym_exchange_bc(mg->blk, S, -1);
-    if (!sym_exchange_smlvl(mg->blk, S)) {
+    ShellOrigins S(L0.hb, 1, (size_t)L0.cells_gh);  // the one-layer level-0 geometry
+    shell_bound_lsm(mg->blk, S);
+    if (periodic) shell_exchange_bc(mg->blk, S, -1, 0);
+    if (!shell_exchange_smlvl(mg->blk, S, -1, 0)) {
         mg->err = "heaviside: edge or corner neighbours are ambiguous (block grid not conforming)";
         return MF3D_ENOTSUP;
     }
     std::vector<uint2> ent;
-    map_entries(S, ent);
+    shell_entries(S, ent);
     std::vector<LsBlock> lb(nb);
     long long mx = 0;
     for (int b = 0; b < nb; ++b) {
@@ -2322,12 +2358,13 @@ int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const
         }
     if ((rc = set_device(mg))) return rc;
     Level &L0 = mg->lv[0];
-    const size_t bytes = sizeof(double) * (size_t)L0.cells_gh;
+    const int G = mg->G;  // the whole caller shell travels
+    const size_t bytes = sizeof(double) * (size_t)mg->cells_w;
     for (int i = 0; i < 8; ++i)
         if (!mg->vf[i] && (i < 7 || su) && (rc = dev_alloc(mg, (void **)&mg->vf[i], bytes))) return rc;
     const double *const *in[7] = {ustar, vstar, wstar, u, v, w, p};
     for (int i = 0; i < 7; ++i) {
-        host_to_packed(mg, in[i], mg->h_io, 1);
+        host_to_packed(mg, in[i], mg->h_io, G);
         HIPCHK(hipMemcpyAsync(mg->vf[i], mg->h_io, bytes, hipMemcpyHostToDevice, mg->stream));
         HIPCHK(hipStreamSynchronize(mg->stream));
     }
@@ -2339,11 +2376,11 @@ int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const
     for (int i = 0; i < 4; ++i) {
         HIPCHK(hipMemcpyAsync(mg->h_io, mg->vf[3 + i], bytes, hipMemcpyDeviceToHost, mg->stream));
         HIPCHK(hipStreamSynchronize(mg->stream));
-        packed_to_host(mg, mg->h_io, out[i], 1);
+        packed_to_host(mg, mg->h_io, out[i], G);
     }
     if (su) {
         hipLaunchKernelGGL(k_pack<double>, dim3(L0.n_in), dim3(kTileX, kTileY), 0, mg->stream, mg->vf[7], mg->d_io,
-                           L0.db, L0.t_in);
+                           mg->d_vw, L0.t_in);
         HIPCHK(hipMemcpyAsync(mg->h_io, mg->d_io, sizeof(double) * mg->cells, hipMemcpyDeviceToHost, mg->stream));
         HIPCHK(hipStreamSynchronize(mg->stream));
         packed_to_host(mg, mg->h_io, su, 0);
@@ -2353,6 +2390,8 @@ int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const
 }
 
 int64_t mf3d_mg_ghosted_cells(const mf3d_mg *mg) { return mg ? mg->lv[0].cells_gh : MF3D_EINVAL; }
+
+int64_t mf3d_mg_shell_cells(const mf3d_mg *mg) { return mg ? mg->cells_w : MF3D_EINVAL; }
 
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel) {
     if (!mg || kernel < MF3D_K_OFF || kernel > MF3D_K_HEAVISIDE) return MF3D_EINVAL;

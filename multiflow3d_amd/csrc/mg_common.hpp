@@ -76,6 +76,18 @@ template <> __device__ __forceinline__ float cconst<float>(const BlockLevel &g, 
     return d == 0 ? g.f0 : (d == 1 ? g.f1 : g.f2);
 }
 
+// s + a*b in the inner cycle's arithmetic.  FP64 rounds the product and
+// the sum separately, as the reference's expression does (bitwise parity);
+// FP32, the mixed form's inner cycle, which only has to converge, contracts
+// them into one fused multiply-add.  Every FP32 sweep kernel accumulates
+// its stencil through this, in the same operand order, so the sweep
+// variants stay bitwise equal to each other.
+template <typename T> __device__ __forceinline__ T madd(T s, T a, T b);
+template <> __device__ __forceinline__ double madd<double>(double s, double a, double b) { return s + a * b; }
+template <> __device__ __forceinline__ float madd<float>(float s, float a, float b) {
+    return __builtin_fmaf(a, b, s);
+}
+
 // The six neighbour coefficients of cell (i,j,k) and ap (mgsolver.for:184-208).
 // Constant form: 1/h^2 with a zero on a physical Neumann face.  Variable
 // form: face arrays cx, cy, cz where cx at (i,j,k) is aw of cell i and cx at

@@ -164,22 +164,22 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
     auto update = [&](const T *P, const T *Pm, const T *Pp, int l, int x, int y, int z, uint32_t o, T r) -> T {
         if (z >= zf0 && z <= zf1) {
             T s = ca * P[l - 1];
-            s = s + ca * P[l + 1];
-            s = s + cb * P[l - LX];
-            s = s + cb * P[l + LX];
-            s = s + cc * Pm[l];
-            s = s + cc * Pp[l];
+            s = madd(s, ca, P[l + 1]);
+            s = madd(s, cb, P[l - LX]);
+            s = madd(s, cb, P[l + LX]);
+            s = madd(s, cc, Pm[l]);
+            s = madd(s, cc, Pp[l]);
             return RCP ? (r - s) * rap0 : (r - s) / ap0;
         }
         T aw, ae, as, an, ab, at, ap;
         const long long q = (long long)(o / W) + g.sz * z;
         coefs<T, VAR>(g, cx, cy, cz, q, x, y, z, aw, ae, as, an, ab, at, ap);
         T s = aw * P[l - 1];
-        s = s + ae * P[l + 1];
-        s = s + as * P[l - LX];
-        s = s + an * P[l + LX];
-        s = s + ab * Pm[l];
-        s = s + at * Pp[l];
+        s = madd(s, ae, P[l + 1]);
+        s = madd(s, as, P[l - LX]);
+        s = madd(s, an, P[l + LX]);
+        s = madd(s, ab, Pm[l]);
+        s = madd(s, at, Pp[l]);
         return RCP ? (r - s) * (T(1) / ap) : (r - s) / ap;
     };
 
@@ -304,6 +304,132 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T 
 #undef MF3D_RBS_STEP
 }
 
+// The zero-start sweep (the first sweep after restriction, or of a cycle
+// from pp = 0): the potential is zero, ghosts included, so the first colour
+// is rhs/ap on every cell and the second colour reads only those values.
+// No potential is loaded: a z-march over the RHS alone that writes every
+// interior cell of pout (with its ghost images on a lone block), bitwise
+// what k_rbs<ZERO> and k_rb3d<ZERO> store (same expressions: (r - 0)/ap,
+// then the second colour's update in the usual operand order).
+//
+// A workgroup owns SW_TX x TY columns of one block and a chunk z0..z1 of
+// planes.  At step c it computes the first-colour values of plane c over
+// the tile plus a 1-cell ring into an LDS ring of 4 planes (zero at
+// second-colour and out-of-block positions), then, after one barrier,
+// stores plane c-1: first-colour cells from LDS, second-colour cells from
+// their six LDS neighbours (planes c-2, c-1, c).  Four slots make one
+// barrier per step enough: the slot written at step c+1 holds plane c-3,
+// which step c does not read, and no thread writes at step c+2 before all
+// have passed the barrier of step c+1.  The RHS loads run 4 steps ahead
+// in 4 register sets (the loop is unrolled by 4, so slots and sets are
+// compile-time).
+template <typename T, bool VAR, bool RCP, int TY>
+__global__ __launch_bounds__(SW_TX *TY) void k_rbz(T *__restrict__ pout, const T *__restrict__ rhs,
+                                                  const T *__restrict__ cx, const T *__restrict__ cy,
+                                                  const T *__restrict__ cz, uint32_t bytes, int own_ghosts,
+                                                  const BlockLevel *__restrict__ bls,
+                                                  const Tile *__restrict__ tiles, const int *__restrict__ done) {
+    if (done && *done) return;
+    constexpr int LX = SW_TX + 2, LY = TY + 2, LXY = LX * LY, NT = SW_TX * TY, NS = 4;
+    constexpr int NRING = 2 * LX + 2 * (LY - 2);
+    static_assert(NRING <= NT, "one ring cell per thread at most");
+    constexpr uint32_t W = sizeof(T);
+    __shared__ T red[NS][LXY];
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b];
+    const auto rs_rhs = make_rsrc(rhs, bytes), rs_out = make_rsrc(pout, bytes);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + SW_TX * ty;
+    const int sy = (int)g.sy;
+    const uint32_t pb = (uint32_t)g.sz * W, ob = (uint32_t)g.off * W;
+    // the own tile cell, and the ring cell of threads below NRING
+    const int x = t.x0 + tx, y = t.y0 + ty;
+    const bool in = x <= g.n0 && y <= g.n1;
+    const int lo = (tx + 1) + LX * (ty + 1);
+    int rlx, rly;
+    {
+        const int q = tid < NRING ? tid : 0;
+        if (q < LX) { rlx = q; rly = 0; }
+        else if (q < 2 * LX) { rlx = q - LX; rly = LY - 1; }
+        else { const int u = q - 2 * LX; rlx = (u & 1) ? LX - 1 : 0; rly = 1 + (u >> 1); }
+    }
+    const bool has_ring = tid < NRING;
+    const int xr = t.x0 - 1 + rlx, yr = t.y0 - 1 + rly, lr = rlx + LX * rly;
+    auto inside = [&](int xx, int yy) { return xx >= 1 && xx <= g.n0 && yy >= 1 && yy <= g.n1; };
+    const bool in_r = has_ring && inside(xr, yr);
+    const uint32_t o_own = ob + (uint32_t)(in ? x + sy * y : sy + 1) * W;
+    const uint32_t o_ring = ob + (uint32_t)(in_r ? xr + sy * yr : sy + 1) * W;
+    const int bix = own_ghosts && in ? ghost_image(x, g.n0, g.neu & kNeuW, g.neu & kNeuE) : -1;
+    const int biy = own_ghosts && in ? ghost_image(y, g.n1, g.neu & kNeuS, g.neu & kNeuN) : -1;
+    // first-colour value of (xx, yy, cc) from its RHS r, 0 elsewhere
+    auto redval = [&](bool ok, int xx, int yy, int cc, T r) -> T {
+        if (!ok || cc < 1 || cc > g.n2 || !((xx + yy + cc) & 1)) return T(0);
+        T aw, ae, as, an, ab, at, ap;
+        coefs<T, VAR>(g, cx, cy, cz, gidx(g, xx, yy, cc), xx, yy, cc, aw, ae, as, an, ab, at, ap);
+        return RCP ? (r - T(0)) * (T(1) / ap) : (r - T(0)) / ap;
+    };
+    auto ld = [&](uint32_t o, int cc) {
+        const int cl = cc < 0 ? 0 : (cc > g.n2 + 1 ? g.n2 + 1 : cc);
+        return bload<T>(rs_rhs, o, (uint32_t)cl * pb);
+    };
+    const int cs = t.z0 - 1;
+    int nst = t.z1 + 2 - cs;  // steps c = z0-1 .. z1+1
+    nst = (nst + NS - 1) / NS * NS;
+    T R[NS], RR[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        R[j] = ld(o_own, cs + j);
+        RR[j] = ld(o_ring, cs + j);
+    }
+    T rprev = T(0);
+    for (int c0 = cs; c0 < cs + nst; c0 += NS) {
+#pragma unroll
+        for (int J = 0; J < NS; ++J) {
+            const int c = c0 + J;
+            const T rcur = R[J], rring = RR[J];
+            R[J] = ld(o_own, c + NS);
+            RR[J] = ld(o_ring, c + NS);
+            red[J][lo] = redval(in, x, y, c, rcur);
+            if (has_ring) red[J][lr] = redval(in_r, xr, yr, c, rring);
+            __syncthreads();
+            const int z = c - 1;
+            if (in && z >= t.z0 && z <= t.z1) {
+                const T *P = red[(J + NS - 1) % NS];
+                T v;
+                if ((x + y + z) & 1) {
+                    v = P[lo];
+                } else {
+                    T aw, ae, as, an, ab, at, ap;
+                    coefs<T, VAR>(g, cx, cy, cz, gidx(g, x, y, z), x, y, z, aw, ae, as, an, ab, at, ap);
+                    T s = aw * P[lo - 1];
+                    s = madd(s, ae, P[lo + 1]);
+                    s = madd(s, as, P[lo - LX]);
+                    s = madd(s, an, P[lo + LX]);
+                    s = madd(s, ab, red[(J + NS - 2) % NS][lo]);
+                    s = madd(s, at, red[J][lo]);
+                    v = RCP ? (rprev - s) * (T(1) / ap) : (rprev - s) / ap;
+                }
+                bstore<T>(rs_out, v, o_own, (uint32_t)z * pb);
+                const int iz = own_ghosts ? ghost_image(z, g.n2, g.neu & kNeuB, g.neu & kNeuT) : -1;
+                if (bix >= 0 || biy >= 0 || iz >= 0) {
+                    auto img = [&](int xx, int yy, int zz) {
+                        bstore<T>(rs_out, v, ob + (uint32_t)(xx + sy * yy) * W, (uint32_t)zz * pb);
+                    };
+                    if (bix >= 0) img(bix, y, z);
+                    if (biy >= 0) img(x, biy, z);
+                    if (bix >= 0 && biy >= 0) img(bix, biy, z);
+                    if (iz >= 0) {
+                        img(x, y, iz);
+                        if (bix >= 0) img(bix, y, iz);
+                        if (biy >= 0) img(x, biy, iz);
+                        if (bix >= 0 && biy >= 0) img(bix, biy, iz);
+                    }
+                }
+            }
+            rprev = rcur;
+        }
+    }
+}
+
 // The same red-black sweep (mgsolver.for:549-601) over 3-D tiles, without
 // a march, for levels where the march's per-step latency dominates (few
 // columns, few planes).  A workgroup loads its R3_TX x R3_TY x R3_TZ tile of
@@ -365,11 +491,11 @@ __global__ __launch_bounds__(R3_NT) void k_rb3d(const T *__restrict__ pin, T *__
         const long long q = g.off + x + (long long)sy * y + g.sz * z;
         coefs<T, VAR>(g, cx, cy, cz, q, x, y, z, aw, ae, as, an, ab, at, ap);
         T s = aw * sp[l - 1];
-        s = s + ae * sp[l + 1];
-        s = s + as * sp[l - LX];
-        s = s + an * sp[l + LX];
-        s = s + ab * sp[l - LXY];
-        s = s + at * sp[l + LXY];
+        s = madd(s, ae, sp[l + 1]);
+        s = madd(s, as, sp[l - LX]);
+        s = madd(s, an, sp[l + LX]);
+        s = madd(s, ab, sp[l - LXY]);
+        s = madd(s, at, sp[l + LXY]);
         return RCP ? (r - s) * (T(1) / ap) : (r - s) / ap;
     };
     // first colour ((x+y+z) odd) on the tile and its ring, interior cells only

@@ -8,6 +8,7 @@
 // wraps onto itself), where the exchange is local to the workgroup.  Same
 // operations, same order and same rounding as the per-level kernels.
 #pragma once
+#include "mg_coarse.hpp"
 #include "mg_common.hpp"
 
 namespace mf3d {
@@ -218,11 +219,68 @@ __device__ __forceinline__ void tail_sweeps(T *sm, const TLv v, const BlockLevel
     }
 }
 
+// FP64 direct solve of the coarsest level inside the tail (one block): the
+// level's RHS in cell order (i fastest) widened into sb, phi = Minv * rhs
+// by one wave per row (coarse_row, as k_coarse_solve computes it), then the
+// ghost fill.
+template <typename T>
+__device__ __forceinline__ void tail_direct(T *sm, const TLv v, const double *__restrict__ cinv, int nc,
+                                            double *sb) {
+    Walk w(threadIdx.x, TAIL_NT, v.n0, v.n1);
+    for (int c = threadIdx.x; c < nc; c += TAIL_NT, w.next(v.n0, v.n1))
+        sb[c] = (double)sm[v.ro + (w.i + 1) + v.sy * (w.j + 1) + v.sz * (w.k + 1)];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int r = wave; r < nc; r += TAIL_NT / 64) {
+        const double s = coarse_row(cinv + (long long)r * nc, sb, nc, lane);
+        if (lane == 0) {
+            const int k = r / (v.n0 * v.n1), rem = r - k * v.n0 * v.n1, j = rem / v.n0, i = rem - j * v.n0;
+            sm[v.po + (i + 1) + v.sy * (j + 1) + v.sz * (k + 1)] = (T)s;
+        }
+    }
+    __syncthreads();
+    tail_exchange<T>(sm, v);
+    __syncthreads();
+}
+
+// The reference's sweeps of the coarsest level computed in FP64 inside an
+// FP32 tail: the level's RHS widened into sd (ghosted layout, after the
+// potential), the potential from zero, n sweeps with FP64 coefficients
+// (cx..cz: the level's FP64 face coefficients under VAR), then every
+// entry, ghosts included, narrowed back.
+template <typename T, bool VAR>
+__device__ __forceinline__ void tail_sweeps_fp64(T *sm, const TLv v, const BlockLevel &g, const double *cx,
+                                                 const double *cy, const double *cz, int n, double *sd) {
+    const int M = v.sz * (v.n2 + 2);
+    TLv vd = v;
+    vd.po = 0;
+    vd.ro = M;
+    for (int c = threadIdx.x; c < M; c += TAIL_NT) {
+        sd[c] = 0.0;
+        sd[M + c] = (double)sm[v.ro + c];
+    }
+    __syncthreads();
+    tail_sweeps<double, VAR>(sd, vd, g, cx, cy, cz, n);
+    for (int c = threadIdx.x; c < M; c += TAIL_NT) sm[v.po + c] = (T)sd[c];
+    __syncthreads();
+}
+
+// cinv (nullptr: sweeps): the coarsest level's inverse operator, nc cells;
+// c64: the coarsest level's sweeps in FP64, with its FP64 coefficients
+// c64c[0..2] under VAR.  Their FP64 staging goes after the level arrays in
+// LDS.
+template <typename T>
+struct TailC64 {
+    int on;
+    const double *cx, *cy, *cz;
+};
+
 template <typename T, bool VAR>
 __global__ __launch_bounds__(TAIL_NT) void k_tail(const BlockLevel *__restrict__ tl, int nb, int nlev,
                                                   const T *__restrict__ rhs_in,
                                                   const T *__restrict__ phi_in, T *__restrict__ phi_out,
                                                   TailArgs<T> co, int nu_pre, int nu_post,
+                                                  const double *__restrict__ cinv, int nc, TailC64<T> c64,
                                                   const int *__restrict__ done) {
     if (done && *done) return;
     extern __shared__ unsigned char smem_raw[];
@@ -258,8 +316,16 @@ __global__ __launch_bounds__(TAIL_NT) void k_tail(const BlockLevel *__restrict__
                                   co.cz[li - 1]);
             __syncthreads();
         }
-        tail_sweeps<T, VAR>(sm, lv[li], tl[li * nb + b], co.cx[li], co.cy[li], co.cz[li],
-                            nu_pre + (li == nlev - 1 ? nu_post : 0));
+        if ((cinv || c64.on) && li == nlev - 1) {
+            const TLv &v = lv[nlev - 1];
+            const int end = v.ro + v.sz * (v.n2 + 2);  // the last level array ends the T arrays
+            double *sb = reinterpret_cast<double *>(smem_raw + ((size_t)end * sizeof(T) + 7) / 8 * 8);
+            if (cinv) tail_direct<T>(sm, v, cinv, nc, sb);
+            else tail_sweeps_fp64<T, VAR>(sm, v, tl[li * nb + b], c64.cx, c64.cy, c64.cz, nu_pre + nu_post, sb);
+        } else {
+            tail_sweeps<T, VAR>(sm, lv[li], tl[li * nb + b], co.cx[li], co.cy[li], co.cz[li],
+                                nu_pre + (li == nlev - 1 ? nu_post : 0));
+        }
     }
     for (int li = nlev - 2; li >= 0; --li) {
         tail_prolong<T>(sm, lv[li], lv[li + 1]);

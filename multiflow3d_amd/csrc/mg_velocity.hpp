@@ -2,9 +2,12 @@
 // update, the projection of the predicted velocity (calvel) fused with its
 // divergence (calmas), and the ghost maps of exchange(1..4) + pbound.
 //
-// Fields u*, v*, w*, u, v, w, p and the cell density live in level-0
-// arenas (one ghost layer, i fastest, blocks one after another).  A face
-// field at index i holds the face east of (north of, above) cell i.
+// Fields u*, v*, w*, u, v, w and p live in shell arenas: every block's
+// array with the caller's G = pl ghost layers, i fastest, blocks one after
+// another (a BlockLevel of the shell geometry addresses coordinates
+// 1-G .. n+G).  The cell density and the V-cycle's RHS live in level-0
+// arenas (one ghost layer).  A face field at index i holds the face east
+// of (north of, above) cell i.
 //
 // Arithmetic follows press.for operand by operand (the library is built
 // with -ffp-contract=off):
@@ -29,7 +32,8 @@ struct VelBlock {
     int lo_src[3];
 };
 
-// p += pp over every ghosted entry (newsolv_mg.for:38-51; ppref is 0).
+// p += pp over every ghosted entry (newsolv_mg.for:38-51; ppref is 0),
+// shell and level-0 layouts equal (one ghost layer).
 template <typename TE>
 __global__ __launch_bounds__(256) void k_vel_pupdate(double *__restrict__ p, const TE *__restrict__ pp,
                                                      long long n, const int *__restrict__ done) {
@@ -37,6 +41,26 @@ __global__ __launch_bounds__(256) void k_vel_pupdate(double *__restrict__ p, con
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
         p[i] = p[i] + (double)pp[i];
+}
+
+// The same over the one-cell box of a wider shell: row tiles of the
+// level-0 geometry (4 whole ghosted rows of one plane), p addressed in the
+// shell geometry.
+template <typename TE>
+__global__ __launch_bounds__(256) void k_vel_pupdate_shell(double *__restrict__ p, const TE *__restrict__ pp,
+                                                           const BlockLevel *__restrict__ bls,
+                                                           const BlockLevel *__restrict__ wbls,
+                                                           const Tile *__restrict__ tiles,
+                                                           const int *__restrict__ done) {
+    if (done && *done) return;
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel g = bls[t.b], gw = wbls[t.b];
+    const int y = t.y0 + threadIdx.y, z = t.z0;
+    if (y > g.n1 + 1) return;
+    for (int x = threadIdx.x; x <= g.n0 + 1; x += blockDim.x) {
+        const long long cw = gidx(gw, x, y, z);
+        p[cw] = p[cw] + (double)pp[gidx(g, x, y, z)];
+    }
 }
 
 // A composed ghost map: entry t copies the value its source held before
@@ -100,20 +124,30 @@ __device__ __forceinline__ double face_val(const FaceOp &f, double dta) {
     return f.stored ? f.s : face_val<LSM>(f.s, f.p0, f.p1, f.d0, f.d1, f.h, dta);
 }
 
-// Operands of face A of block g at arena offset c, index i along A.
+// Operands of face A of a block at index i along A: shell offset c of the
+// fields (geometry gw), level-0 offset cn of the density (geometry g).
 template <int A, bool LSM>
-__device__ __forceinline__ FaceOp own_face(const VelFields &F, const BlockLevel &g, int ie, long long c, int i) {
+__device__ __forceinline__ FaceOp own_face(const VelFields &F, const BlockLevel &g, const BlockLevel &gw, int ie,
+                                           long long c, long long cn, int i) {
     FaceOp f;
-    const long long st = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
+    const long long st = A == 0 ? 1 : (A == 1 ? gw.sy : gw.sz);
+    const long long sn = A == 0 ? 1 : (A == 1 ? g.sy : g.sz);
     f.h = A == 0 ? g.h0 : (A == 1 ? g.h1 : g.h2);
     f.stored = i > ie;
     const double *sv = f.stored ? (A == 0 ? F.u : (A == 1 ? F.v : F.w)) : (A == 0 ? F.us : (A == 1 ? F.vs : F.ws));
     f.s = sv[c];
     f.p0 = F.p[c];
     f.p1 = F.p[c + st];
-    f.d0 = LSM ? F.dens[c] : 1.0;
-    f.d1 = LSM ? F.dens[c + st] : 1.0;
+    f.d0 = LSM ? F.dens[cn] : 1.0;
+    f.d1 = LSM ? F.dens[cn + sn] : 1.0;
     return f;
+}
+
+template <int A, bool LSM>
+__device__ __forceinline__ FaceOp own_face_at(const VelFields &F, const BlockLevel &g, const BlockLevel &gw, int ie,
+                                              int x, int y, int z) {
+    const int i = A == 0 ? x : (A == 1 ? y : z);
+    return own_face<A, LSM>(F, g, gw, ie, gidx(gw, x, y, z), gidx(g, x, y, z), i);
 }
 
 // The ghost face below the first cell of component A at transverse cell
@@ -122,12 +156,12 @@ __device__ __forceinline__ FaceOp own_face(const VelFields &F, const BlockLevel 
 // face before a wall.
 template <int A, bool LSM>
 __device__ __forceinline__ FaceOp low_face(const VelFields &F, const BlockLevel *__restrict__ bls,
-                                           const VelBlock *__restrict__ vbs, const BlockLevel &g, int b, int i1,
-                                           int i2) {
+                                           const BlockLevel *__restrict__ wbls, const VelBlock *__restrict__ vbs,
+                                           const BlockLevel &gw, int b, int i1, int i2) {
     const int s = vbs[b].lo_src[A];
     if (s < 0) {
         FaceOp f;
-        const long long c0 = A == 0 ? gidx(g, 0, i1, i2) : (A == 1 ? gidx(g, i1, 0, i2) : gidx(g, i1, i2, 0));
+        const long long c0 = A == 0 ? gidx(gw, 0, i1, i2) : (A == 1 ? gidx(gw, i1, 0, i2) : gidx(gw, i1, i2, 0));
         f.s = (A == 0 ? F.u : (A == 1 ? F.v : F.w))[c0];
         f.p0 = f.p1 = 0.0;
         f.d0 = f.d1 = 1.0;
@@ -135,10 +169,11 @@ __device__ __forceinline__ FaceOp low_face(const VelFields &F, const BlockLevel 
         f.stored = true;
         return f;
     }
-    const BlockLevel gs = bls[s];
+    const BlockLevel gs = bls[s], gsw = wbls[s];
     const int ns = A == 0 ? gs.n0 : (A == 1 ? gs.n1 : gs.n2);
-    const long long cs = A == 0 ? gidx(gs, ns, i1, i2) : (A == 1 ? gidx(gs, i1, ns, i2) : gidx(gs, i1, i2, ns));
-    return own_face<A, LSM>(F, gs, vbs[s].ie[A], cs, ns);
+    if (A == 0) return own_face_at<A, LSM>(F, gs, gsw, vbs[s].ie[A], ns, i1, i2);
+    if (A == 1) return own_face_at<A, LSM>(F, gs, gsw, vbs[s].ie[A], i1, ns, i2);
+    return own_face_at<A, LSM>(F, gs, gsw, vbs[s].ie[A], i1, i2, ns);
 }
 
 // calvel fused with calmas over the finest level.  A persistent grid walks
@@ -164,8 +199,9 @@ constexpr int VEL_TZ_MAX = 32;
 template <typename TR, bool LSM, bool SU64, int TY>
 __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
     VelFields F, TR *__restrict__ rhs, double *__restrict__ su64, double *__restrict__ sink,
-    const SolveState *__restrict__ st, const BlockLevel *__restrict__ bls, const VelBlock *__restrict__ vbs,
-    const Tile *__restrict__ tiles, int ntiles, double *__restrict__ partials, const int *__restrict__ done) {
+    const SolveState *__restrict__ st, const BlockLevel *__restrict__ bls, const BlockLevel *__restrict__ wbls,
+    const VelBlock *__restrict__ vbs, const Tile *__restrict__ tiles, int ntiles, double *__restrict__ partials,
+    const int *__restrict__ done) {
     if (done && *done) return;
     constexpr int NT = SW_TX * TY;
     __shared__ double sW[VEL_TZ_MAX][TY];      // west faces of the tile's first column
@@ -177,7 +213,7 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
     double rm = 0.0;
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const Tile t = tiles[ti];
-        const BlockLevel g = bls[t.b];
+        const BlockLevel g = bls[t.b], gw = wbls[t.b];
         const VelBlock vb = vbs[t.b];
         const int nz = t.z1 - t.z0 + 1;
         __syncthreads();  // the previous tile's halo reads are done
@@ -185,8 +221,8 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
         for (int q = tid; q < TY * nz; q += NT) {
             const int yy = t.y0 + q % TY, zz = t.z0 + q / TY;
             if (yy <= g.n1) {
-                const FaceOp f = t.x0 == 1 ? low_face<0, LSM>(F, bls, vbs, g, t.b, yy, zz)
-                                           : own_face<0, LSM>(F, g, vb.ie[0], gidx(g, t.x0 - 1, yy, zz), t.x0 - 1);
+                const FaceOp f = t.x0 == 1 ? low_face<0, LSM>(F, bls, wbls, vbs, gw, t.b, yy, zz)
+                                           : own_face_at<0, LSM>(F, g, gw, vb.ie[0], t.x0 - 1, yy, zz);
                 sW[q / TY][q % TY] = face_val<LSM>(f, dta);
             }
         }
@@ -194,8 +230,8 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
         for (int q = tid; q < SW_TX * nz; q += NT) {
             const int xx = t.x0 + q % SW_TX, zz = t.z0 + q / SW_TX;
             if (xx <= g.n0) {
-                const FaceOp f = t.y0 == 1 ? low_face<1, LSM>(F, bls, vbs, g, t.b, xx, zz)
-                                           : own_face<1, LSM>(F, g, vb.ie[1], gidx(g, xx, t.y0 - 1, zz), t.y0 - 1);
+                const FaceOp f = t.y0 == 1 ? low_face<1, LSM>(F, bls, wbls, vbs, gw, t.b, xx, zz)
+                                           : own_face_at<1, LSM>(F, g, gw, vb.ie[1], xx, t.y0 - 1, zz);
                 sS[q / SW_TX][q % SW_TX] = face_val<LSM>(f, dta);
             }
         }
@@ -209,8 +245,8 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
         const double *nsrc = nstored ? F.v : F.vs;
         double wB;  // bottom face of the first plane
         {
-            const FaceOp f = t.z0 == 1 ? low_face<2, LSM>(F, bls, vbs, g, t.b, xc, yc)
-                                       : own_face<2, LSM>(F, g, vb.ie[2], gidx(g, xc, yc, t.z0 - 1), t.z0 - 1);
+            const FaceOp f = t.z0 == 1 ? low_face<2, LSM>(F, bls, wbls, vbs, gw, t.b, xc, yc)
+                                       : own_face_at<2, LSM>(F, g, gw, vb.ie[2], xc, yc, t.z0 - 1);
             wB = face_val<LSM>(f, dta);
         }
         // operands of one plane; p and rho at the cell come from the plane below
@@ -218,28 +254,28 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
             double us, vs, ws, pe, pn, pz, de, dn, dz;
         };
         auto load = [&](Ops &o, int z) {
-            const long long c = gidx(g, xc, yc, z);
+            const long long c = gidx(gw, xc, yc, z);
             const double *tsrc = z > vb.ie[2] ? F.w : F.ws;
             o.us = esrc[c];
             o.vs = nsrc[c];
             o.ws = tsrc[c];
             o.pe = F.p[c + 1];
-            o.pn = F.p[c + g.sy];
-            o.pz = F.p[c + g.sz];
+            o.pn = F.p[c + gw.sy];
+            o.pz = F.p[c + gw.sz];
             if (LSM) {
-                o.de = F.dens[c + 1];
-                o.dn = F.dens[c + g.sy];
-                o.dz = F.dens[c + g.sz];
+                const long long cn = gidx(g, xc, yc, z);
+                o.de = F.dens[cn + 1];
+                o.dn = F.dens[cn + g.sy];
+                o.dz = F.dens[cn + g.sz];
             }
         };
         double pc, dc = 1.0;
         {
-            const long long c = gidx(g, xc, yc, t.z0);
-            pc = F.p[c];
-            if (LSM) dc = F.dens[c];
+            pc = F.p[gidx(gw, xc, yc, t.z0)];
+            if (LSM) dc = F.dens[gidx(g, xc, yc, t.z0)];
         }
         auto step = [&](const Ops &o, int z) {
-            const long long c = gidx(g, xc, yc, z);
+            const long long c = gidx(gw, xc, yc, z), cn = gidx(g, xc, yc, z);
             double pe = __shfl_down(pc, 1);
             double de = LSM ? __shfl_down(dc, 1) : 1.0;
             pe = eown ? o.pe : pe;
@@ -260,7 +296,7 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
             *((act && !nstored) ? F.v + c : my_sink) = vN;
             *((act && !tstored) ? F.w + c : my_sink) = wT;
             if (SU64) *(act ? su64 + c : my_sink) = su;
-            else *(act ? rhs + c : (TR *)my_sink) = (TR)su;
+            else *(act ? rhs + cn : (TR *)my_sink) = (TR)su;
             rm = act ? fmax(rm, fabs(vol)) : rm;
             wB = wT;
             pc = o.pz;
@@ -285,6 +321,55 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
     sm[tid] = rm;
     __syncthreads();
     for (int w = NT / 2; w > 0; w >>= 1) {
+        if (tid < w) sm[tid] = fmax(sm[tid], sm[tid + w]);
+        __syncthreads();
+    }
+    if (tid == 0) partials[blockIdx.x] = sm[0];
+}
+
+// calmas, 4th-order branch (differencing = 2, press.for:13-36), after
+// calvel and exchange(1..3) have refreshed the faces it reads (u(i+1) of
+// the last cell is a ghost face, u(i-2) of the first the second ghost
+// layer): fact = sum over the axes of (-u(i+1) + 27 u(i) - 27 u(i-1) +
+// u(i-2)) / (24 dx), in the reference's operand order; su = fact/dt into
+// the V-cycle's RHS (inner precision TR, level-0 geometry) or, SU64, into
+// su64 (shell geometry); per workgroup max|fact*dx*dy*dz| into partials.
+// Workgroups stride over the level-0 cell tiles (64 x 4 cells of a plane).
+template <typename TR, bool SU64>
+__global__ __launch_bounds__(256) void k_calmas4(VelFields F, TR *__restrict__ rhs, double *__restrict__ su64,
+                                                 const SolveState *__restrict__ st,
+                                                 const BlockLevel *__restrict__ bls,
+                                                 const BlockLevel *__restrict__ wbls, const Tile *__restrict__ tiles,
+                                                 int ntiles, double *__restrict__ partials,
+                                                 const int *__restrict__ done) {
+    if (done && *done) return;
+    __shared__ double sm[256];
+    const double dt = st->dt;
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    double rm = 0.0;
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile t = tiles[ti];
+        const BlockLevel g = bls[t.b], gw = wbls[t.b];
+        const int x = t.x0 + threadIdx.x, y = t.y0 + threadIdx.y, z = t.z0;
+        if (x > g.n0 || y > g.n1) continue;
+        const long long c = gidx(gw, x, y, z);
+        const long long sy = gw.sy, sz = gw.sz;
+        const double dx = g.h0, dy = g.h1, dz = g.h2;
+        const double fx = (((-F.u[c + 1] + 27.0 * F.u[c]) - 27.0 * F.u[c - 1]) + F.u[c - 2]) / (24.0 * dx);
+        const double fy =
+            (((-F.v[c + sy] + 27.0 * F.v[c]) - 27.0 * F.v[c - sy]) + F.v[c - 2 * sy]) / (24.0 * dy);
+        const double fz =
+            (((-F.w[c + sz] + 27.0 * F.w[c]) - 27.0 * F.w[c - sz]) + F.w[c - 2 * sz]) / (24.0 * dz);
+        const double fact = (fx + fy) + fz;
+        const double vol = ((fact * dx) * dy) * dz;
+        if (SU64) su64[c] = fact / dt;
+        else rhs[gidx(g, x, y, z)] = (TR)(fact / dt);
+        rm = fmax(rm, fabs(vol));
+    }
+    if (!partials) return;
+    sm[tid] = rm;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) sm[tid] = fmax(sm[tid], sm[tid + w]);
         __syncthreads();
     }

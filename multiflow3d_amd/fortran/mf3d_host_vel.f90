@@ -9,7 +9,9 @@
 ! lsm = 2 the density comes from a level set first, as the reference's
 ! step does (flosol.for:114-117): mf3d_mg_heaviside replaces heaviside
 ! (lsm.for:1456-1509) and the coeff of newsolv_mg.for:12-14.  The outer
-! ghost layer is filled with a marker the library must leave alone.
+! ghost layer starts as a marker: newsolv's exchanges write parts of it
+! as the reference's do (whole periodic planes, 3-deep corner and edge
+! cubes), heaviside leaves it alone.
 !
 ! Usage: mf3d_host_vel <control file>
 !   nbx nby nbz levels lsm maxcy
@@ -22,9 +24,10 @@
 !     phi under lsm = 2), each the block's one-ghost-layer box (n+2)^3,
 !     f64, i fastest
 !   output file
-! Output (f64 stream): iters, rmax(0:iters), then per block the one-layer
-! boxes of p, u, v, w, the interior of su, (lsm = 2) the one-layer boxes
-! of dens and mu, and the count of outer-layer entries that changed.
+! Output (f64 stream): iters, rmax(0:iters), then per block the whole
+! arrays (pl layers) of p, u, v, w, the interior of su, (lsm = 2) the
+! one-layer boxes of dens and mu, and the count of outer-layer entries of
+! dens and mu that changed.
 program mf3d_host_vel
   use iso_c_binding
   use mf3d_mg_f
@@ -161,13 +164,12 @@ program mf3d_host_vel
   write(u) dble(iters)
   write(u) hist(1:iters + 1)
   do ib = 1, nb
-    call write_box(u, dom(ib)%p, 1)
-    call write_box(u, dom(ib)%u, 1)
-    call write_box(u, dom(ib)%v, 1)
-    call write_box(u, dom(ib)%w, 1)
+    call write_box(u, dom(ib)%p, pl)
+    call write_box(u, dom(ib)%u, pl)
+    call write_box(u, dom(ib)%v, pl)
+    call write_box(u, dom(ib)%w, pl)
     call write_box(u, dom(ib)%su, 0)
-    changed = outer_changed(dom(ib)%p) + outer_changed(dom(ib)%u) + outer_changed(dom(ib)%v) &
-              + outer_changed(dom(ib)%w)
+    changed = 0
     if (lsmf == 2) then
       call write_box(u, dom(ib)%dens, 1)
       call write_box(u, dom(ib)%mu, 1)

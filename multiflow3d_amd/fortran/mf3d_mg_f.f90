@@ -32,6 +32,8 @@ module mf3d_mg_f
     real(c_double)     :: eps
     integer(c_int32_t) :: min_cycles, precision, variable_coef, device
     integer(c_int32_t) :: coarse_op = 0, krylov = 0  ! reference hierarchy, defect correction
+    integer(c_int32_t) :: coarse_solve = 0           ! direct in mixed precision, smoothing in FP64
+    integer(c_int32_t) :: differencing = 1, pl_ex = 0 ! control.cin's differencing and pl_ex
   end type
 
   public :: mf3d_mg_create, mf3d_mg_set_density, mf3d_mg_vcycle, mf3d_mg_solve

@@ -70,9 +70,13 @@ def _descs(blocks: Sequence[Block], ghost: int):
     return arr
 
 
+COARSE_SOLVES = {"default": _lib.MF3D_COARSEST_DEFAULT, "inner": _lib.MF3D_COARSEST_INNER,
+                 "direct": _lib.MF3D_COARSEST_DIRECT}
+
+
 def make_cfg(nblocks: int, levels: int, *, nu_pre=2, nu_post=2, maxcy=30, eps=1e-8,
              min_cycles=1, precision="fp64", variable_coef=False, device=0, smoother=3,
-             coarse_op="reference", krylov=False):
+             coarse_op="reference", krylov=False, coarse_solve="default", differencing=1, pl_ex=0):
     c = _lib.Cfg()
     c.nblocks, c.levels, c.smoother = nblocks, levels, smoother
     c.nu_pre, c.nu_post, c.maxcy = nu_pre, nu_post, maxcy
@@ -82,6 +86,9 @@ def make_cfg(nblocks: int, levels: int, *, nu_pre=2, nu_post=2, maxcy=30, eps=1e
     c.device = device
     c.coarse_op = COARSE_OPS[coarse_op] if isinstance(coarse_op, str) else int(coarse_op)
     c.krylov = int(bool(krylov))
+    c.coarse_solve = COARSE_SOLVES[coarse_solve] if isinstance(coarse_solve, str) else int(coarse_solve)
+    c.differencing = int(differencing)
+    c.pl_ex = int(pl_ex)
     return c
 
 
@@ -97,24 +104,34 @@ def estimate_bytes(grid_or_blocks, levels: int, **kw) -> int:
 class PressureMultigrid:
     """One solver handle: device-resident level hierarchy for a block set."""
 
-    GHOST = 1   # caller arrays carry one ghost layer, as the reference's pp (pl = 1)
+    GHOST = 1   # default ghost layers of caller arrays (the reference's pl with LMR 1, pl_ex 0)
 
     def __init__(self, grid_or_blocks, levels: int, *, nu_pre: int = 2, nu_post: int = 2,
                  maxcy: int = 30, eps: float = 1e-8, min_cycles: int = 1,
                  precision: str = "fp64", variable_coef: bool = False, device: int = 0,
-                 smoother: int = 3, coarse_op: str = "reference", krylov: bool = False):
+                 smoother: int = 3, coarse_op: str = "reference", krylov: bool = False,
+                 coarse_solve: str = "default", differencing: int = 1, pl_ex: int = 0,
+                 ghost: Optional[int] = None):
+        """ghost: layers of every caller array (the reference's pl; default 1).
+        The velocity form (newsolv) works on the whole shell: exchange(1..3)
+        refreshes pl_ex extra layers, differencing = 2 selects the 4th-order
+        calmas.  coarse_solve: "default" (the reference's smoothing of the
+        coarsest level, in FP64 in both precisions), "inner" (that smoothing
+        in the inner precision) or "direct" (an exact FP64 solve)."""
         self.blocks: List[Block] = (grid_or_blocks.blocks if isinstance(grid_or_blocks, BlockGrid)
                                     else list(grid_or_blocks))
         self.levels = levels
         self.maxcy = maxcy
         self.variable_coef = variable_coef
+        self.ghost = int(self.GHOST if ghost is None else ghost)
         self.cfg = make_cfg(len(self.blocks), levels, nu_pre=nu_pre, nu_post=nu_post,
                             maxcy=maxcy, eps=eps, min_cycles=min_cycles, precision=precision,
                             variable_coef=variable_coef, device=device, smoother=smoother,
-                            coarse_op=coarse_op, krylov=krylov)
+                            coarse_op=coarse_op, krylov=krylov, coarse_solve=coarse_solve,
+                            differencing=differencing, pl_ex=pl_ex)
         L = _lib.lib()
         h = ctypes.c_void_p()
-        rc = L.mf3d_mg_create(ctypes.byref(self.cfg), _descs(self.blocks, self.GHOST), ctypes.byref(h))
+        rc = L.mf3d_mg_create(ctypes.byref(self.cfg), _descs(self.blocks, self.ghost), ctypes.byref(h))
         self._h = h
         if rc != 0:
             msg = L.mf3d_mg_last_error(h).decode() if h.value else ""
@@ -152,19 +169,26 @@ class PressureMultigrid:
 
     # ------------------------------------------------------------------
     def _ghosted(self, arrays, copy_in=True):
+        """Caller arrays with self.ghost layers: given so, or from interiors
+        (nz, ny, nx) or one-layer boxes (nz+2, ny+2, nx+2), zero-padded."""
         out = []
+        G = self.ghost
         for b, a in zip(self.blocks, arrays):
             nx, ny, nz = b.n
             a = np.asarray(a, dtype=np.float64)
-            if a.shape == (nz + 2, ny + 2, nx + 2):
+            if a.shape == (nz + 2 * G, ny + 2 * G, nx + 2 * G):
                 out.append(np.ascontiguousarray(a))
-            elif a.shape == (nz, ny, nx):
-                g = np.zeros((nz + 2, ny + 2, nx + 2))
+                continue
+            g = np.zeros((nz + 2 * G, ny + 2 * G, nx + 2 * G))
+            if a.shape == (nz, ny, nx):
                 if copy_in:
-                    g[1:-1, 1:-1, 1:-1] = a
-                out.append(g)
+                    g[G:-G, G:-G, G:-G] = a
+            elif a.shape == (nz + 2, ny + 2, nx + 2):
+                if copy_in:
+                    g[G - 1:g.shape[0] - G + 1, G - 1:g.shape[1] - G + 1, G - 1:g.shape[2] - G + 1] = a
             else:
                 raise ValueError(f"block array shape {a.shape} does not match extent {b.n}")
+            out.append(g)
         return out
 
     @staticmethod
@@ -189,6 +213,11 @@ class PressureMultigrid:
                                                    ctypes.c_void_p(stream_ptr or None))
         if rc != 0:
             raise MGError(f"coeff failed ({rc}): {self._err()}")
+
+    @property
+    def shell_cells(self) -> int:
+        """Length of newsolv_device's fields (blocks' arrays with their ghost layers)."""
+        return int(_lib.lib().mf3d_mg_shell_cells(self._h))
 
     def heaviside(self, phi, densl: float, densg: float, mul: float, mug: float):
         """heaviside (lsm.for:1456-1509) + bound_lsm(18/19), then coeff from
@@ -218,10 +247,11 @@ class PressureMultigrid:
     def mgkcyc(self, rhs, pp):
         """One V-cycle on ``rhs``; ``pp`` (ghosted arrays) is updated in place and returned."""
         r = self._ghosted(rhs)
+        G = self.ghost
         for b, a in zip(self.blocks, pp):
-            if a.shape != (b.n[2] + 2, b.n[1] + 2, b.n[0] + 2) or a.dtype != np.float64 \
+            if a.shape != (b.n[2] + 2 * G, b.n[1] + 2 * G, b.n[0] + 2 * G) or a.dtype != np.float64 \
                     or not a.flags.c_contiguous:
-                raise ValueError("pp blocks must be C-contiguous float64 with one ghost layer")
+                raise ValueError(f"pp blocks must be C-contiguous float64 with {G} ghost layer(s)")
         rc = _lib.lib().mf3d_mg_vcycle(self._h, self._ptrs(r), self._ptrs(list(pp)))
         if rc != 0:
             raise MGError(f"mgkcyc failed ({rc}): {self._err()}")
@@ -249,7 +279,8 @@ class PressureMultigrid:
                                       ctypes.byref(iters),
                                       self._hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         res = self._finish(rc, rmax, iters)
-        out = [a[1:-1, 1:-1, 1:-1].copy() for a in pg]
+        G = self.ghost
+        out = [a[G:-G, G:-G, G:-G].copy() for a in pg]
         if p is not None:
             for dst, src in zip(p, out):
                 if dst.shape == src.shape:
@@ -265,8 +296,9 @@ class PressureMultigrid:
 
     def _vel_arrays(self, arrays, name, writable):
         out = []
+        G = self.ghost
         for b, a in zip(self.blocks, arrays):
-            shp = (b.n[2] + 2, b.n[1] + 2, b.n[0] + 2)
+            shp = (b.n[2] + 2 * G, b.n[1] + 2 * G, b.n[0] + 2 * G)
             if a.shape != shp or a.dtype != np.float64 or not a.flags.c_contiguous or \
                     (writable and not a.flags.writeable):
                 raise ValueError(f"{name} blocks must be C-contiguous float64 arrays shaped {shp}")
@@ -280,9 +312,11 @@ class PressureMultigrid:
         """newsolv_mg in the reference's velocity form (newsolv_mg.for:12-58):
         calvel/calmas, then V-cycles on su (sup under LSM), p += pp,
         exchange(4)/pbound, calvel/calmas, until rmax < eps after two cycles.
-        All fields are per-block ghosted arrays (nz+2, ny+2, nx+2); u, v, w and
-        p are updated in place (faces on walls and untouched ghosts keep their
-        values), su (optional, same shape) gets the last right-hand side in its
+        All fields are per-block arrays with the handle's ghost layers
+        (nz+2g, ny+2g, nx+2g); u, v, w and p are updated in place over the
+        interior faces and the ghost shell the reference's exchanges write
+        (faces on walls and untouched ghosts keep their values), su
+        (optional, same shape) gets the last right-hand side in its
         interior.  Returns a SolveResult whose hist is rmax per cycle (index 0
         before the first).  Under LSM, coeff(density) must have received the
         ghosted density."""
@@ -304,8 +338,9 @@ class PressureMultigrid:
     def newsolv_device(self, ptrs, dt: float, alfapr: float, eps: float, su_ptr: int = 0,
                        stream_ptr: int = 0, warn: bool = True) -> SolveResult:
         """Device-resident velocity form: ptrs = (ustar, vstar, wstar, u, v, w, p)
-        device addresses of ghosted_cells doubles each (e.g. torch tensors'
-        data_ptr()); su_ptr optional."""
+        device addresses of shell_cells doubles each (every block's array
+        with the handle's ghost layers; e.g. torch tensors' data_ptr());
+        su_ptr optional."""
         if len(ptrs) != 7:
             raise ValueError("ptrs: ustar, vstar, wstar, u, v, w, p")
         rmax = ctypes.c_double(0.0)

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from multiflow3d_amd.build import FORTRAN_HOST, FORTRAN_HOST_VEL
+from oracle import velocity as V
 from oracle.oracle import OracleMG
 
 from .cases import case, inputs
@@ -69,7 +70,10 @@ def test_fortran_host_matches_oracle(tmp_path, name):
 def test_fortran_velocity_host_matches_reference_fixture(tmp_path, name):
     """A Fortran host with pl = 2 arrays calls mf3d_mg_newsolv: its p, u, v,
     w (one-ghost-layer box), su and rmax history equal the compiled
-    reference's golden vectors, and the outer ghost layer is untouched."""
+    reference's golden vectors, and the whole two-layer shell (outer layer
+    started as a marker) equals the oracle's, whose wide-shell exchanges
+    are pinned by the reference (whole periodic planes and 3-deep corner
+    cubes write parts of the outer layer)."""
     g, m, inp, exp = load_case(name)
     fin = tmp_path / "in.bin"
     parts = []
@@ -93,18 +97,29 @@ def test_fortran_velocity_host_matches_reference_fixture(tmp_path, name):
     k = int(raw[0])
     assert k == exp["iters"]
     assert np.array_equal(raw[1:k + 2], exp["rmax"])
+    # the oracle on the host's two-layer inputs (outer layer the marker -7)
+    def wide(a):
+        x = np.full(tuple(d + 2 for d in a.shape), -7.0)
+        x[1:-1, 1:-1, 1:-1] = a
+        return x
+    winp = {key: [wide(a) for a in inp[key]] for key in ("ustar", "vstar", "wstar", "p0")}
+    ko, ho, po, uo, vo, wo, _ = V.newsolv_velocity(
+        g, m["levels"], winp["ustar"], winp["vstar"], winp["wstar"], winp["p0"], m["eps"], m["maxcy"], m["dt"],
+        m["alfapr"], lsm=m["lsm"], dens=inp["dens"])
+    assert ko == k
     o = k + 2
     for b, blk in enumerate(g.blocks):
         nx, ny, nz = blk.n
-        gsz = (nx + 2) * (ny + 2) * (nz + 2)
-        for key in ("p", "u", "v", "w"):
-            a = raw[o:o + gsz].reshape(nz + 2, ny + 2, nx + 2)
-            o += gsz
-            assert np.array_equal(a, exp[key][b]), (key, b)
+        wsz = (nx + 4) * (ny + 4) * (nz + 4)
+        for key, orc in (("p", po), ("u", uo), ("v", vo), ("w", wo)):
+            a = raw[o:o + wsz].reshape(nz + 4, ny + 4, nx + 4)
+            o += wsz
+            assert np.array_equal(a[1:-1, 1:-1, 1:-1], exp[key][b]), (key, b)
+            assert np.array_equal(a, orc[b]), (key, b, np.argwhere(a != orc[b])[:5])
         isz = nx * ny * nz
         assert np.array_equal(raw[o:o + isz].reshape(nz, ny, nx), exp["su"][b])
         o += isz
-        assert raw[o] == 0.0, "the library wrote outside the one-cell ghost layer"
+        assert raw[o] == 0.0
         o += 1
 
 
@@ -114,7 +129,8 @@ def test_fortran_levelset_host_matches_reference_fixture(tmp_path, name):
     """The reference's LSM step order from a Fortran host with pl = 2
     arrays: mf3d_mg_heaviside on phi (dens, mu out), then mf3d_mg_newsolv.
     dens and mu to the last-place bar of the level-set tests, the chain
-    bitwise where the densities are; the outer ghost layer untouched."""
+    bitwise where the densities are (one-layer box); the outer ghost layer
+    of dens and mu untouched."""
     from .test_gpu_levelset import _check_dens, _rel
     from .test_oracle_levelset import load_ls
     g, m, a = load_ls(name)
@@ -145,16 +161,17 @@ def test_fortran_levelset_host_matches_reference_fixture(tmp_path, name):
     for blk in g.blocks:
         nx, ny, nz = blk.n
         gsz = (nx + 2) * (ny + 2) * (nz + 2)
+        wsz = (nx + 4) * (ny + 4) * (nz + 4)
         for key in ("p", "u", "v", "w"):
-            got[key].append(raw[o:o + gsz].reshape(nz + 2, ny + 2, nx + 2))
-            o += gsz
+            got[key].append(raw[o:o + wsz].reshape(nz + 4, ny + 4, nx + 4)[1:-1, 1:-1, 1:-1])
+            o += wsz
         isz = nx * ny * nz
         got["su"].append(raw[o:o + isz].reshape(nz, ny, nx))
         o += isz
         for key in ("dens", "mu"):
             got[key].append(raw[o:o + gsz].reshape(nz + 2, ny + 2, nx + 2))
             o += gsz
-        assert raw[o] == 0.0, "the library wrote outside the one-cell ghost layer"
+        assert raw[o] == 0.0, "heaviside wrote outside the one-cell ghost layer"
         o += 1
     nd = _check_dens(got["dens"], a["dens"], densg, densl)
     _check_dens(got["mu"], a["mu"], mug, mul)

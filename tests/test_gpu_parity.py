@@ -134,6 +134,7 @@ SWITCHES = {
     "spatial_tiles": {"MF3D_XCD_ORDER": "0"},
     "default_depths": {"MF3D_AUTOTUNE": "0"},
     "pinned_depth": {"MF3D_SWEEP_TZ": "4"},
+    "zero_start_by_full_sweep": {"MF3D_ZERO_MARCH": "0"},
     "all_off": {"MF3D_TAIL": "0", "MF3D_OWN_GHOSTS": "0", "MF3D_XCD_ORDER": "0", "MF3D_AUTOTUNE": "0"},
 }
 
@@ -143,7 +144,8 @@ SWITCHES = {
 def test_layout_switches_keep_results_bitwise(name, switch, monkeypatch):
     """Each environment switch of the library changes only how the work is
     laid out (tail kernel, who writes a lone block's ghosts, tile order,
-    chunk depths): FP64 stays bitwise the oracle's."""
+    chunk depths, which kernel runs the zero-start sweep): FP64 stays
+    bitwise the oracle's."""
     for k, v in SWITCHES[switch].items():
         monkeypatch.setenv(k, v)
     grid, L, lsm = case(name)

@@ -2,7 +2,10 @@
 
 - Every request's pressure is bit for bit the same in a mix of overlapping
   solves as when solved alone on a fresh handle (deterministic kernels; the
-  stream, slot and neighbours on the device do not enter the arithmetic).
+  stream, slot and neighbours on the device do not enter the arithmetic):
+  a random small mix, and the full size mix (every size of the serving
+  workload, 32^3 .. 256^3, with both coefficient kinds and all three
+  tolerances).
 - HBM-budgeted admission: with room for one 64^3 hierarchy, requests still
   all complete and the pool never holds more than the budget.
 """
@@ -19,7 +22,7 @@ def _alone(req, precision, inputs):
     g = S.Inputs.grid(req.n)
     with PressureMultigrid(g, S.levels_for(req.n), maxcy=60, eps=req.tol, precision=precision,
                            variable_coef=req.two_phase, coarse_op="face" if req.two_phase else "reference",
-                           krylov=req.two_phase) as mg:
+                           krylov=req.two_phase, coarse_solve="direct") as mg:
         if req.two_phase:
             mg.coeff(inputs.density(req.n))
         f = inputs.rhs(req.n, req.seed, req.two_phase)
@@ -37,6 +40,30 @@ def test_mix_equals_alone(precision):
         assert [r.rid for r in out] == [q.rid for q in reqs]
         for q, r in zip(reqs, out):
             p, res = _alone(q, precision, srv.inputs)
+            assert r.status == 0 and res.status == 0 and r.rho <= q.tol
+            assert r.iters == res.iters, (q, r.iters, res.iters)
+            assert np.array_equal(r.p.to_host(), p), f"request {q.rid} differs between the mix and alone"
+    finally:
+        srv.close()
+
+
+def test_full_size_mix_equals_alone():
+    """Every serving size (32 .. 256 per axis) in both coefficient kinds,
+    tolerances cycling 1e-6, 1e-8, 1e-10, all submitted at once to 4 slots
+    (mixed precision): each converges to its tolerance and is bitwise its
+    solve alone."""
+    reqs = []
+    for i, (n, tp) in enumerate([(n, tp) for n in S.SIZES for tp in (False, True)]):
+        reqs.append(S.Request(rid=i, arrival=0.0, n=n, tol=S.TOLERANCES[i % 3], two_phase=tp,
+                              seed=7000 + i))
+    assert {q.n for q in reqs} == set(S.SIZES) and sum(q.two_phase for q in reqs) == len(reqs) // 2
+    srv = S.Server(slots=4, precision="mixed")
+    try:
+        out = srv.run(reqs, realtime=False)
+        assert [r.rid for r in out] == [q.rid for q in reqs]
+        for q, r in zip(reqs, out):
+            print(f"[serving full mix] n={q.n} two_phase={q.two_phase} tol={q.tol}", flush=True)
+            p, res = _alone(q, "mixed", srv.inputs)
             assert r.status == 0 and res.status == 0 and r.rho <= q.tol
             assert r.iters == res.iters, (q, r.iters, res.iters)
             assert np.array_equal(r.p.to_host(), p), f"request {q.rid} differs between the mix and alone"
