@@ -26,15 +26,28 @@ W_SEED = 20261202  # the workloads' base seed
 METRIC = "MG solves/s to 1e-8 resid (mixed vs FP64), HBM GB/s % peak, p50/p99 ms"
 
 
-def algorithmic_bytes_per_cycle(n_cells_levels, w, c=0):
-    """Streaming model per V(2,2) cycle: sweeps 4(3+c) w N_l, restriction and
-    prolongation (4.25+c) w N_l below the coarsest, FP64 outer (32+2w+8c) N_1."""
+def algorithmic_bytes_per_cycle(n_cells_levels, w, c=0, gcr=False):
+    """Streaming model per V(2,2) cycle (w: bytes of the inner precision, c:
+    coefficient arrays per cell, 3 under variable coefficients).  On every
+    level 4 sweeps of (3+c) w N_l less w N_l for the first, which starts
+    from zero and reads no potential; restriction and prolongation
+    (4.25+c) w N_l below the coarsest.  Finest-level FP64 outer step:
+    (24+2w+8c) N_0 for the defect correction (p, f, e read; p, r written;
+    FP64 coefficients), (80+3w+24c) N_0 for GCR(1)'s three passes
+    (A z . w; d = z - beta d, w = A d; p += alpha d, r = f - A p)."""
     nu = 4
     L = len(n_cells_levels)
-    b_inner = sum(nu * (3 + c) * n * w for n in n_cells_levels)
+    b_inner = sum((nu * (3 + c) - 1) * n * w for n in n_cells_levels)
     b_inner += sum((4.25 + c) * n_cells_levels[l] * w for l in range(L - 1))
-    b_outer = (32 + 2 * w + 8 * c) * n_cells_levels[0]
-    return b_inner + b_outer
+    per = (80 + 3 * w + 24 * c) if gcr else (24 + 2 * w + 8 * c)
+    return b_inner + per * n_cells_levels[0]
+
+
+def sweep_bytes_per_launch(w, c=0):
+    """Finest-level sweep launches of a V(2,2) cycle: three full sweeps
+    (read phi, the RHS and c coefficient arrays, write phi: (3+c) w per
+    cell) and one from zero ((2+c) w), (11+4c)/4 w per cell on average."""
+    return (3 * (3 + c) + (2 + c)) / 4.0 * w
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rbs_256cubed_L7_traffic.md")
@@ -70,7 +83,10 @@ def measured_traffic(form, path=TRAFFIC_FILE, key="traffic_bytes_per_launch:"):
     return None
 
 
-def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_kernel):
+def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_kernel, dens=None,
+            warn=True, **kw):
+    """Device-resident solves of one problem: dens (per-block cell density)
+    selects the variable-coefficient operator; kw goes to the handle."""
     import torch
 
     from multiflow3d_amd import PressureMultigrid
@@ -80,11 +96,14 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     f_dev = torch.from_numpy(np.ascontiguousarray(np.concatenate([a.ravel() for a in f_host]))).to(dev)
     p_dev = torch.zeros_like(f_dev)
     torch.cuda.synchronize()  # the solver runs on its own non-blocking stream
-    mg = PressureMultigrid(grid, levels, maxcy=maxcy, eps=tol, precision=precision)
+    mg = PressureMultigrid(grid, levels, maxcy=maxcy, eps=tol, precision=precision,
+                           variable_coef=dens is not None, **kw)
+    if dens is not None:
+        mg.coeff(dens)
     if time_kernel:
         mg.time_kernel(_lib.MF3D_K_SMOOTH)
     for _ in range(warmup):
-        mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr())
+        mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr(), warn=warn)
     torch.cuda.synchronize()
     if time_kernel:
         mg.time_kernel(_lib.MF3D_K_SMOOTH)  # resets the counters
@@ -94,7 +113,7 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
     t0 = time.perf_counter()
     for _ in range(steps):
         ts = time.perf_counter()
-        res = mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr())
+        res = mg.solve_device(f_dev.data_ptr(), p_dev.data_ptr(), warn=warn)
         lat.append((time.perf_counter() - ts) * 1e3)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -107,6 +126,122 @@ def run_gpu(grid, levels, f_host, precision, steps, warmup, tol, maxcy, time_ker
                hook_replay=replay if time_kernel else None)
     mg.close()
     return out
+
+
+def run_pair(grid, levels, f_host, steps, warmup, tol, maxcy, label, dens=None, **kw):
+    """FP64 and mixed solves of one problem, each with the finest sweep's
+    launches timed inside the timed region (graph replay) where the runtime
+    captures the timing events, else timed eagerly and the solve rate taken
+    from a second, graph-replayed pass."""
+    out = {}
+    for prec in ("fp64", "mixed"):
+        r = run_gpu(grid, levels, f_host, prec, steps, warmup, tol, maxcy, True, dens=dens, **kw)
+        if not r["graph_replay"]:
+            t = run_gpu(grid, levels, f_host, prec, steps, warmup, tol, maxcy, False, dens=dens, **kw)
+            r.update(ms=t["ms"], lat=t["lat"], graph_replay=t["graph_replay"])
+        print(f"# {label} {prec}: {r['ms']:.3f} ms/solve, {r['iters']} cycles, rho={r['rho']:.3e}",
+              file=sys.stderr, flush=True)
+        out[prec] = r
+    return out["fp64"], out["mixed"]
+
+
+def rel_l2_mean_removed(a, b):
+    a = a - a.mean()
+    b = b - b.mean()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def sweep_roofline(r, w, n0, c=0, traffic=None, traffic_source=None, kernel="k_rbs"):
+    """Roofline of the finest-level sweep from the hook's launch times:
+    sweep_bytes_per_launch(w, c) x n0 interior cells per launch."""
+    if not r["launches"]:
+        return None
+    avg_s = r["kernel_ms"] / r["launches"] * 1e-3
+    per_launch = sweep_bytes_per_launch(w, c) * n0
+    achieved = per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic, "traffic_source": traffic_source if traffic else None,
+            "kernel": f"{kernel} (fused red-black sweep, finest level)", "avg_us": round(avg_s * 1e6, 2),
+            "launches_timed": r["launches"], "bytes_per_launch": int(per_launch),
+            "timed_with": "HIP events on the solver's stream around each launch, inside the timed "
+                          "region (graph replay)" if r["hook_replay"] else
+                          "HIP events on the solver's stream around each launch, over the same "
+                          "solves launched eagerly, the step time taken from a second pass with "
+                          "graph replay (events recorded by a graph cannot be timed here)"}
+
+
+def solve_leg(r, w, levels_cells, c=0, gcr=False, roof=None):
+    """The per-precision record of one timed solve."""
+    b = algorithmic_bytes_per_cycle(levels_cells, w, c, gcr) * r["iters"]
+    gbps = b / (r["ms"] * 1e-3) / 1e9
+    return {"solves_per_s": round(1e3 / r["ms"], 4), "ms_per_solve": round(r["ms"], 4),
+            "cycles": r["iters"], "final_rho": r["rho"], "status": r["status"],
+            "alg_GBps": round(gbps, 1), "alg_frac_of_peak": round(gbps / HBM_PEAK_GBPS, 4),
+            "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+            "p99_ms": round(float(np.percentile(r["lat"], 99)), 3),
+            "graph_replay": r["graph_replay"], "roofline": roof}
+
+
+def run_uneven(steps, warmup, tol, maxcy):
+    """The uneven 2 x 2 x 2 block grid: per-axis extents drawn from
+    {96, 128, 160, 192} by SplitMix64 of seed BASE_SEED + 3, channel
+    boundaries, 6 levels, all blocks of a level in one launch."""
+    from multiflow3d_amd import topology as T
+    from multiflow3d_amd import workloads as W
+
+    ex, ey, ez = W.uneven_extents(W.BASE_SEED + 3)
+    gn = [sum(ex), sum(ey), sum(ez)]
+    grid = T.block_grid(ex, ey, ez, T.CHANNEL_BC, [1.0 / g for g in gn])
+    L = 6
+    f = W.split_blocks(grid, W.global_rhs(grid, W.BASE_SEED + 3))
+    cells = [sum(int(np.prod([x >> l for x in b.n])) for b in grid.blocks) for l in range(L)]
+    r64, rmx = run_pair(grid, L, f, steps, warmup, tol, maxcy, "uneven 2x2x2")
+    n0 = cells[0]
+    return {"workload": f"2 x 2 x 2 blocks, extents x {ex}, y {ey}, z {ez} (seed BASE_SEED+3), "
+                        f"periodic x/z, Neumann y, V(2,2) RBGS, {L} levels, to {tol:g} relative residual, "
+                        f"ghosts by the on-device region-table exchange",
+            "cells": n0, "levels": L, "blocks": [list(b.n) for b in grid.blocks],
+            "fp64": solve_leg(r64, 8, cells, roof=sweep_roofline(r64, 8, n0)),
+            "mixed": solve_leg(rmx, 4, cells, roof=sweep_roofline(rmx, 4, n0)),
+            "mixed_rel_l2_vs_fp64": rel_l2_mean_removed(rmx["p"], r64["p"]),
+            "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3)}
+
+
+def run_two_phase(steps, warmup, tol, maxcy, n=384):
+    """The n^3 two-phase problem: sphere r = 0.2 at the centre, density
+    1000:1 smoothed over 1.5 dx, RHS scaled by the cell volume; face-
+    restricted coarse operators, GCR(1) outer iteration, 7 levels.  Also
+    the reference's own form (its coarse operators, defect correction) in
+    FP64 for maxcy cycles, to show where it stops."""
+    from multiflow3d_amd import topology as T
+    from multiflow3d_amd import workloads as W
+
+    grid = T.single_block(n, [1.0 / n] * 3, T.CHANNEL_BC)
+    L = 7
+    f = W.split_blocks(grid, W.global_rhs(grid, W.BASE_SEED + 4, fv_volume=True))
+    dens = W.split_blocks(grid, W.sphere_density(grid))
+    cells = [(n >> l) ** 3 for l in range(L)]
+    kw = dict(coarse_op="face", krylov=True)
+    r64, rmx = run_pair(grid, L, f, steps, warmup, tol, maxcy, f"{n}^3 two-phase", dens=dens, **kw)
+    ref = run_gpu(grid, L, f, "fp64", 1, 0, tol, maxcy, False, dens=dens, warn=False)
+    print(f"# {n}^3 two-phase reference form fp64: {ref['iters']} cycles, rho={ref['rho']:.3e}",
+          file=sys.stderr, flush=True)
+    n0 = cells[0]
+    return {"workload": f"single {n}^3 block, periodic x/z, Neumann y, div((1/rho) grad p), sphere r 0.2, "
+                        f"rho 1000:1 over 1.5 dx, face-restricted coarse operators, GCR(1) outer, "
+                        f"V(2,2) RBGS, {L} levels, to {tol:g} relative residual",
+            "cells": n0, "levels": L,
+            "fp64": solve_leg(r64, 8, cells, 3, True, sweep_roofline(r64, 8, n0, 3)),
+            "mixed": solve_leg(rmx, 4, cells, 3, True, sweep_roofline(rmx, 4, n0, 3)),
+            "mixed_rel_l2_vs_fp64": rel_l2_mean_removed(rmx["p"], r64["p"]),
+            "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+            "reference_form_fp64": {
+                "cycles": ref["iters"], "final_rho": ref["rho"], "status": ref["status"],
+                "ms_per_solve": round(ref["ms"], 4),
+                "note": "the reference's variable-coefficient V-cycle (coarse operators from the "
+                        "finest spacing, defect correction) stalls on this problem: it stops "
+                        f"near rho = {ref['rho']:.2e}, not at {tol:g}"}}
 
 
 def run_velocity(grid, levels, precision, steps, warmup, rel_eps, maxcy, time_kernel):
@@ -315,6 +450,10 @@ def main():
                     help="skip the shipped-deck leg (80 blocks, one level, ADI smoother)")
     ap.add_argument("--no-levelset", action="store_true",
                     help="skip the level-set density update (heaviside + coeff) leg")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven 2x2x2 block-grid leg")
+    ap.add_argument("--no-two-phase", action="store_true",
+                    help="skip the 384^3 two-phase (1000:1) leg")
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests per arrival rate in the serving sweep (0: skip it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -336,24 +475,7 @@ def main():
     f_host = W.split_blocks(grid, W.global_rhs(grid, seed))
     levels_cells = [(n >> l) ** 3 for l in range(L)]
 
-    r64 = run_gpu(grid, L, f_host, "fp64", args.steps, args.warmup, args.tol, args.maxcy, True)
-    if not r64["graph_replay"]:
-        # the runtime refused to capture the timing events: keep the hook's
-        # kernel times, take the solve rate from graph replay without it
-        t = run_gpu(grid, L, f_host, "fp64", args.steps, args.warmup, args.tol, args.maxcy, False)
-        r64.update(ms=t["ms"], lat=t["lat"], graph_replay=t["graph_replay"])
-    print(f"# fp64: {r64['ms']:.3f} ms/solve, {r64['iters']} cycles, rho={r64['rho']:.3e}", file=sys.stderr, flush=True)
-    rmx = run_gpu(grid, L, f_host, "mixed", args.steps, args.warmup, args.tol, args.maxcy, True)
-    if not rmx["graph_replay"]:
-        t = run_gpu(grid, L, f_host, "mixed", args.steps, args.warmup, args.tol, args.maxcy, False)
-        rmx.update(ms=t["ms"], lat=t["lat"], graph_replay=t["graph_replay"])
-    print(f"# mixed: {rmx['ms']:.3f} ms/solve, {rmx['iters']} cycles, rho={rmx['rho']:.3e}", file=sys.stderr, flush=True)
-
-    def rel_l2_mean_removed(a, b):
-        a = a - a.mean()
-        b = b - b.mean()
-        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-
+    r64, rmx = run_pair(grid, L, f_host, args.steps, args.warmup, args.tol, args.maxcy, f"{n}^3")
     mixed_err = rel_l2_mean_removed(rmx["p"], r64["p"])
 
     if args.calib_copy > 0:
@@ -368,35 +490,14 @@ def main():
         del a, b
 
     # Dominant kernel: k_rbs, one fused red-black sweep (both colours) of the
-    # finest level.  Algorithmic bytes: a sweep reads phi and the RHS and
-    # writes phi once per interior cell, 3 w N; the first sweep of each
-    # cycle starts from phi = 0 and reads only the RHS, 2 w N.  V(2,2) runs
-    # 4 finest sweeps per cycle, one of them from zero: 2.75 w N per launch
-    # on average over the launches the hook times.
+    # finest level; sweep_bytes_per_launch gives its algorithmic bytes.
     N0 = levels_cells[0]
+    src = ("profiles/k_rbs_256cubed_L7_traffic.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per launch, "
+           "calibrated on a 1 GiB copy)")
 
     def roofline(r, w):
-        if not r["launches"]:
-            return None
-        avg_s = r["kernel_ms"] / r["launches"] * 1e-3
-        per_launch = 2.75 * w * N0
-        achieved = per_launch / avg_s / 1e9
         traffic = measured_traffic("fp64" if w == 8 else "mixed") if (n, L) == (256, 7) else None
-        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                "traffic": traffic,
-                "traffic_source": "profiles/k_rbs_256cubed_L7_traffic.md (rocprofv3 FETCH_SIZE x2 + "
-                                  "WRITE_SIZE per launch, calibrated on a 1 GiB copy)" if traffic else None,
-                "kernel": "k_rbs (fused red-black sweep, finest level)", "avg_us": round(avg_s * 1e6, 2),
-                "launches_timed": r["launches"], "bytes_per_launch": int(per_launch),
-                "timed_with": "HIP events on the solver's stream around each launch, inside the timed "
-                              "region (graph replay)" if r["hook_replay"] else
-                              "HIP events on the solver's stream around each launch, over the same "
-                              "solves launched eagerly, the step time taken from a second pass with "
-                              "graph replay (events recorded by a graph cannot be timed here)"}
-
-    b64 = algorithmic_bytes_per_cycle(levels_cells, 8) * r64["iters"]
-    bmx = algorithmic_bytes_per_cycle(levels_cells, 4) * rmx["iters"]
+        return sweep_roofline(r, w, N0, 0, traffic, src)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -520,9 +621,10 @@ def main():
                                f"dx 0.005, periodic x, walls S/N/B, slip top) with its multigrid settings: "
                                f"1 level, ADI-TDMA (mg_itrsch 2), irestr 2, iproln 1, 30 cycles per call, "
                                f"velocity form, u* seeded, p from 0")
+    uneven = None if args.no_uneven else run_uneven(args.steps, args.warmup, args.tol, args.maxcy)
+    two_phase = None if args.no_two_phase else run_two_phase(args.steps, args.warmup, args.tol, args.maxcy)
     serving = run_serving(args.serving_requests) if args.serving_requests > 0 else None
 
-    lat = np.array(rmx["lat"])
     line = {
         "metric": METRIC,
         "value": round(1e3 / rmx["ms"], 4),
@@ -541,23 +643,14 @@ def main():
                    "cells": N0, "levels": L, "tol": args.tol},
         "roofline": roofline(rmx, 4),
         "cpu_baseline": cpu,
-        "fp64": {"solves_per_s": round(1e3 / r64["ms"], 4), "ms_per_solve": round(r64["ms"], 4),
-                 "cycles": r64["iters"], "final_rho": r64["rho"],
-                 "alg_GBps": round(b64 / (r64["ms"] * 1e-3) / 1e9, 1),
-                 "alg_frac_of_peak": round(b64 / (r64["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                 "graph_replay": r64["graph_replay"],
-                 "roofline": roofline(r64, 8)},
-        "mixed": {"solves_per_s": round(1e3 / rmx["ms"], 4), "ms_per_solve": round(rmx["ms"], 4),
-                  "cycles": rmx["iters"], "final_rho": rmx["rho"],
-                  "alg_GBps": round(bmx / (rmx["ms"] * 1e-3) / 1e9, 1),
-                  "alg_frac_of_peak": round(bmx / (rmx["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                  "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                  "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                  "rel_l2_vs_fp64": mixed_err, "graph_replay": rmx["graph_replay"]},
+        "fp64": solve_leg(r64, 8, levels_cells, roof=roofline(r64, 8)),
+        "mixed": dict(solve_leg(rmx, 4, levels_cells), rel_l2_vs_fp64=mixed_err),
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
         "velocity_form": velocity,
         "levelset_update": levelset,
         "shipped_deck": shipped,
+        "uneven_blocks": uneven,
+        "two_phase": two_phase,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -179,9 +179,10 @@ struct mf3d_mg {
     double *c64_phi[2] = {nullptr, nullptr}, *c64_rhs = nullptr;
     CCell *d_ccell = nullptr;
     double *d_cinv = nullptr, *d_ccol = nullptr;
-    // the zero-start sweep of the march levels runs k_rbz (the RHS alone);
-    // MF3D_ZERO_MARCH=0 keeps k_rbs from a zero potential (bitwise the same)
-    bool zmarch = true;
+    // the zero-start sweep of the march levels: 1 a march over the RHS
+    // alone (k_rbz), 0 the full sweep from a zero potential (k_rbs);
+    // MF3D_ZERO_SWEEP picks one, both bitwise the same
+    int zmode = 1;
     // the velocity form's fields on the caller's ghost shell: G layers
     // (every block's `ghost`), per-block shell geometry, total length
     int G = 1;
@@ -569,7 +570,7 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
                            L.t_sw, done);
         return;
     }
-    if (ZERO && mg->zmarch) {  // from zero: the march over the RHS alone
+    if (ZERO && mg->zmode == 1) {  // from zero: the march over the RHS alone
         if (L.sw_ty == 8)
             hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pout,
                                (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
@@ -1297,8 +1298,12 @@ int adi_coef_build(mf3d_mg *mg) {
 int build(mf3d_mg *mg) {
     const mf3d_mg_cfg &c = mg->cfg;
     {
-        const char *e = std::getenv("MF3D_ZERO_MARCH");
-        mg->zmarch = !(e && e[0] == '0');
+        const char *e = std::getenv("MF3D_ZERO_SWEEP");
+        mg->zmode = e ? std::atoi(e) : 1;
+        if (mg->zmode < 0 || mg->zmode > 1) {
+            mg->err = "MF3D_ZERO_SWEEP must be 0 or 1";
+            return MF3D_EINVAL;
+        }
     }
     const int L = c.levels, nb = c.nblocks;
     const bool mixed = c.precision == MF3D_MIXED;

@@ -78,12 +78,13 @@ __device__ __forceinline__ void tail_colour(T *sm, const TLv v, const BlockLevel
         T aw, ae, as, an, ab, at, ap;
         coefs<T, VAR>(g, cx, cy, cz, g.off + q, i, j, k, aw, ae, as, an, ab, at, ap);
         T s = aw * P[q - 1];
-        s = s + ae * P[q + 1];
-        s = s + as * P[q - v.sy];
-        s = s + an * P[q + v.sy];
-        s = s + ab * P[q - v.sz];
-        s = s + at * P[q + v.sz];
-        P[q] = (R[q] - s) / ap;
+        s = madd(s, ae, P[q + 1]);
+        s = madd(s, as, P[q - v.sy]);
+        s = madd(s, an, P[q + v.sy]);
+        s = madd(s, ab, P[q - v.sz]);
+        s = madd(s, at, P[q + v.sz]);
+        // FP64 divides as the reference does; the FP32 inner cycle multiplies by 1/ap
+        P[q] = sizeof(T) == 8 ? (R[q] - s) / ap : (R[q] - s) * (T(1) / ap);
     }
 }
 
