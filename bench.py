@@ -183,6 +183,38 @@ def solve_leg(r, w, levels_cells, c=0, gcr=False, roof=None):
             "graph_replay": r["graph_replay"], "roofline": roof}
 
 
+def direct_variants(grid, levels, f_host, steps, warmup, tol, maxcy, p64, cells, c=0, gcr=False, dens=None,
+                    **kw):
+    """The same problem with the coarsest level solved exactly in FP64
+    (coarse_solve = "direct") in both precisions: solves/s, cycles and the
+    pressure's distance from the FP64 reference-form solve p64.  Returns
+    the records and the pressures."""
+    out, ps = {}, {}
+    for prec, w in (("fp64", 8), ("mixed", 4)):
+        r = run_gpu(grid, levels, f_host, prec, steps, warmup, tol, maxcy, False, dens=dens,
+                    coarse_solve="direct", **kw)
+        print(f"# direct coarse {prec}: {r['ms']:.3f} ms/solve, {r['iters']} cycles, rho={r['rho']:.3e}",
+              file=sys.stderr, flush=True)
+        out[prec] = dict(solve_leg(r, w, cells, c, gcr), rel_l2_vs_fp64=rel_l2_mean_removed(r["p"], p64))
+        ps[prec] = r["p"]
+    return out, ps
+
+
+def accuracy(grid, levels, f_host, pressures, dens=None, tol=1e-13, maxcy=200, **kw):
+    """Distance of each solve's pressure (mean removed) from the FP64 solve
+    taken to tol (coarsest level solved exactly): at a 1e-8 relative
+    residual, what is left in the pressure is mostly the smoothest error
+    component, which a residual norm weights by the operator's smallest
+    eigenvalue, so two solves that both stop at 1e-8 can differ by far
+    more than 1e-8."""
+    r = run_gpu(grid, levels, f_host, "fp64", 1, 0, tol, maxcy, False, dens=dens, warn=False,
+                coarse_solve="direct", **kw)
+    print(f"# converged fp64: {r['iters']} cycles, rho={r['rho']:.3e}", file=sys.stderr, flush=True)
+    return {"reference": f"FP64, coarsest level solved exactly, to {tol:g} relative residual "
+                         f"({r['iters']} cycles, rho {r['rho']:.2e})",
+            "rel_l2_vs_converged": {k: rel_l2_mean_removed(p, r["p"]) for k, p in pressures.items()}}
+
+
 def run_uneven(steps, warmup, tol, maxcy):
     """The uneven 2 x 2 x 2 block grid: per-axis extents drawn from
     {96, 128, 160, 192} by SplitMix64 of seed BASE_SEED + 3, channel
@@ -198,6 +230,9 @@ def run_uneven(steps, warmup, tol, maxcy):
     cells = [sum(int(np.prod([x >> l for x in b.n])) for b in grid.blocks) for l in range(L)]
     r64, rmx = run_pair(grid, L, f, steps, warmup, tol, maxcy, "uneven 2x2x2")
     n0 = cells[0]
+    direct, pd = direct_variants(grid, L, f, steps, warmup, tol, maxcy, r64["p"], cells)
+    acc = accuracy(grid, L, f, {"fp64": r64["p"], "mixed": rmx["p"], "fp64_direct_coarse": pd["fp64"],
+                                "mixed_direct_coarse": pd["mixed"]})
     return {"workload": f"2 x 2 x 2 blocks, extents x {ex}, y {ey}, z {ez} (seed BASE_SEED+3), "
                         f"periodic x/z, Neumann y, V(2,2) RBGS, {L} levels, to {tol:g} relative residual, "
                         f"ghosts by the on-device region-table exchange",
@@ -205,7 +240,8 @@ def run_uneven(steps, warmup, tol, maxcy):
             "fp64": solve_leg(r64, 8, cells, roof=sweep_roofline(r64, 8, n0)),
             "mixed": solve_leg(rmx, 4, cells, roof=sweep_roofline(rmx, 4, n0)),
             "mixed_rel_l2_vs_fp64": rel_l2_mean_removed(rmx["p"], r64["p"]),
-            "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3)}
+            "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+            "coarsest_cells": cells[-1], "direct_coarse": direct, "accuracy": acc}
 
 
 def run_two_phase(steps, warmup, tol, maxcy, n=384):
@@ -228,6 +264,10 @@ def run_two_phase(steps, warmup, tol, maxcy, n=384):
     print(f"# {n}^3 two-phase reference form fp64: {ref['iters']} cycles, rho={ref['rho']:.3e}",
           file=sys.stderr, flush=True)
     n0 = cells[0]
+    direct, pd = direct_variants(grid, L, f, steps, warmup, tol, maxcy, r64["p"], cells, 3, True, dens=dens, **kw)
+    acc = accuracy(grid, L, f, {"fp64": r64["p"], "mixed": rmx["p"], "fp64_direct_coarse": pd["fp64"],
+                                "mixed_direct_coarse": pd["mixed"], "reference_form_fp64": ref["p"]},
+                   dens=dens, **kw)
     return {"workload": f"single {n}^3 block, periodic x/z, Neumann y, div((1/rho) grad p), sphere r 0.2, "
                         f"rho 1000:1 over 1.5 dx, face-restricted coarse operators, GCR(1) outer, "
                         f"V(2,2) RBGS, {L} levels, to {tol:g} relative residual",
@@ -236,6 +276,7 @@ def run_two_phase(steps, warmup, tol, maxcy, n=384):
             "mixed": solve_leg(rmx, 4, cells, 3, True, sweep_roofline(rmx, 4, n0, 3)),
             "mixed_rel_l2_vs_fp64": rel_l2_mean_removed(rmx["p"], r64["p"]),
             "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+            "coarsest_cells": cells[-1], "direct_coarse": direct, "accuracy": acc,
             "reference_form_fp64": {
                 "cycles": ref["iters"], "final_rho": ref["rho"], "status": ref["status"],
                 "ms_per_solve": round(ref["ms"], 4),
@@ -454,6 +495,9 @@ def main():
                     help="skip the uneven 2x2x2 block-grid leg")
     ap.add_argument("--no-two-phase", action="store_true",
                     help="skip the 384^3 two-phase (1000:1) leg")
+    ap.add_argument("--only", choices=("uneven", "two_phase"), default=None,
+                    help="run one workload leg alone and print its record (for profiling it apart "
+                         "from the others)")
     ap.add_argument("--serving-requests", type=int, default=64,
                     help="requests per arrival rate in the serving sweep (0: skip it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -466,6 +510,11 @@ def main():
               "(independent solves shard across GPUs by request)", file=sys.stderr)
         sys.exit(2)
 
+    if args.only:
+        leg = {"uneven": run_uneven, "two_phase": run_two_phase}[args.only]
+        print(json.dumps({"leg": args.only, **leg(args.steps, args.warmup, args.tol, args.maxcy)}), flush=True)
+        return
+
     from multiflow3d_amd import topology as T
     from multiflow3d_amd import workloads as W
 
@@ -477,6 +526,7 @@ def main():
 
     r64, rmx = run_pair(grid, L, f_host, args.steps, args.warmup, args.tol, args.maxcy, f"{n}^3")
     mixed_err = rel_l2_mean_removed(rmx["p"], r64["p"])
+    acc = accuracy(grid, L, f_host, {"fp64": r64["p"], "mixed": rmx["p"]})
 
     if args.calib_copy > 0:
         import torch
@@ -646,6 +696,7 @@ def main():
         "fp64": solve_leg(r64, 8, levels_cells, roof=roofline(r64, 8)),
         "mixed": dict(solve_leg(rmx, 4, levels_cells), rel_l2_vs_fp64=mixed_err),
         "speedup_mixed_vs_fp64": round(r64["ms"] / rmx["ms"], 3),
+        "accuracy": acc,
         "velocity_form": velocity,
         "levelset_update": levelset,
         "shipped_deck": shipped,
