@@ -29,6 +29,8 @@ class Block:
     per: List[int] = field(default_factory=lambda: [-1] * 6)   # per_ip,per_in,per_jp,per_jn,per_kp,per_kn
     bc: List[int] = field(default_factory=lambda: [NEUMANN] * 6)
     origin: List[int] = field(default_factory=lambda: [0, 0, 0])  # global cell offset
+    rdiv: int = 1                     # refinement ratio (infodom.cin column 2); n and h are
+                                      # at the block's own resolution
 
     @property
     def cells(self) -> int:
@@ -43,6 +45,14 @@ class BlockGrid:
     bc: List[int]
     h: List[float]
     blocks: List[Block]
+
+    @property
+    def rdiv(self) -> List[int]:
+        return [b.rdiv for b in self.blocks]
+
+    @property
+    def refined(self) -> bool:
+        return any(b.rdiv > 1 for b in self.blocks)
 
     @property
     def shape(self):
@@ -107,6 +117,24 @@ def block_grid(ex: Sequence[int], ey: Sequence[int], ez: Sequence[int],
     return BlockGrid(ex=ex, ey=ey, ez=ez, bc=bc, h=list(map(float, h)), blocks=blocks)
 
 
+def refined_grid(ex: Sequence[int], ey: Sequence[int], ez: Sequence[int], bc: Sequence[int],
+                 h: Sequence[float], rdiv: Sequence[int]) -> BlockGrid:
+    """The tensor grid of block_grid with block i refined rdiv[i] times
+    (localparameters.for:255-257): extents x rdiv, spacing / rdiv.  ex, ey,
+    ez, h and the blocks' origins stay in unrefined cells."""
+    g = block_grid(ex, ey, ez, bc, h)
+    if len(rdiv) != len(g.blocks):
+        raise ValueError("one rdiv per block")
+    for blk, r in zip(g.blocks, rdiv):
+        r = int(r)
+        if r < 1 or r & (r - 1):
+            raise ValueError("rdiv must be a power of 2")
+        blk.rdiv = r
+        blk.n = [n * r for n in blk.n]
+        blk.h = [x / r for x in blk.h]
+    return g
+
+
 def single_block(n, h, bc) -> BlockGrid:
     n = list(n) if hasattr(n, "__len__") else [n, n, n]
     return block_grid([n[0]], [n[1]], [n[2]], bc, h)
@@ -117,9 +145,11 @@ CHANNEL_BC = [PERIODIC, PERIODIC, NEUMANN, NEUMANN, PERIODIC, PERIODIC]
 
 
 def max_levels(grid: BlockGrid) -> int:
-    """Largest level count such that every block extent divides by 2**(L-1)
-    (localparameters.for:201-206)."""
-    L = 1
-    while all((n % (1 << L)) == 0 for blk in grid.blocks for n in blk.n):
+    """Largest global level count ngrd_gl such that every block extent
+    divides by 2**(own levels - 1), a block of rdiv r having log2(r) levels
+    more than an unrefined one (localparameters.for:33-56,201-206)."""
+    top = max(b.rdiv for b in grid.blocks).bit_length() - 1
+    L = top + 1
+    while all((n % (1 << (L - top + (b.rdiv.bit_length() - 1)))) == 0 for b in grid.blocks for n in b.n):
         L += 1
     return L

@@ -90,6 +90,20 @@ def global_rhs(grid: BlockGrid, seed: int, fv_volume: bool = False, dt: float = 
     return f
 
 
+def block_rhs(grid: BlockGrid, seed: int):
+    """Per-block RHS of a grid whose blocks need not share a resolution
+    (refined blocks): uniform(-1, 1) SplitMix64 draws of ``seed``, block by
+    block in block order (z, y, x within a block), less the mean over all
+    cells."""
+    out, o = [], 0
+    for blk in grid.blocks:
+        m = blk.cells
+        out.append(uniform_pm1(seed, o, m).reshape(blk.n[2], blk.n[1], blk.n[0]))
+        o += m
+    mean = sum(float(a.sum()) for a in out) / o
+    return [np.ascontiguousarray(a - mean) for a in out]
+
+
 def split_blocks(grid: BlockGrid, field: np.ndarray):
     """Cut a global (z, y, x) field into per-block contiguous (z, y, x) arrays."""
     out = []

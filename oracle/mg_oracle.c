@@ -28,14 +28,32 @@
  * reduction stays serial, so the results are bitwise those of the serial
  * build.
  *
- * Scope notes (what is not restated):
- *  - refined-block (rdv unequal) exchange branches and the injection branch
- *    for blocks with fewer levels (mgsolver.for:719-732, 807-819);
- *  - the explicit corner/edge messages of exchangepp.for:433-1093: on a
- *    conforming block grid they carry the same values the three full-face
- *    passes already deliver (each face pass copies whole ghosted planes,
- *    so x -> y -> z propagates edges and corners).  Multi-block fixtures
- *    from the compiled reference pin this.
+ * Refined blocks (rdiv > 1, localparameters.for:26-56): a block with
+ * rdiv = 2^r has r more levels of its own; at a global level beyond its own
+ * count it repeats its coarsest grid, and its restriction and prolongation
+ * become injection (mgsolver.for:672-680,719-732,807-819).  Across a face
+ * whose two blocks differ in rdv at a level, the face pass of exchangepp
+ * sends 0.125 x the 8 fine cells a coarse ghost covers (fine to coarse,
+ * exchangepp.for:163-172,1110-1120) and 2 x the coarse cell each of the 4
+ * fine ghosts over it lies against (coarse to fine, :173-188); the coarse
+ * side stores the interior rows of its ghost face, the fine side
+ * (received + 3 phi_1 - phi_2) / 4 (:4046-4053,4257-4262).  The reference's
+ * fine side also rewrites the edge rows of that ghost face, from receive
+ * buffer entries no sender filled (left over from earlier exchanges); here
+ * they are left as they are (orc_set_poison writes NaN there instead, to
+ * show whether a result depends on them).  Scope: constant coefficients,
+ * a periodic axis only where a block wraps onto itself (no partner block:
+ * exchbc_mgpp has no 2:1 branch), face neighbours within a factor 2 in
+ * rdiv, and no two
+ * blocks that touch along an edge or at a corner only (the reference's
+ * corner and edge messages, exchangepp.for:433-1093, have 2:1 branches of
+ * their own).
+ *
+ * Scope notes (what is not restated): the explicit corner/edge messages of
+ * exchangepp.for:433-1093: on a conforming block grid they carry the same
+ * values the three full-face passes already deliver (each face pass copies
+ * whole ghosted planes, so x -> y -> z propagates edges and corners).
+ * Multi-block fixtures from the compiled reference pin this.
  */
 #include "mg_oracle.h"
 
@@ -47,6 +65,10 @@ enum { FW = 0, FE = 1, FS = 2, FN = 3, FB = 4, FT = 5 };
 
 struct orc_mg {
     int nb, L, lsm, nu_pre, nu_post, periodic;
+    int lin;            /* ngrid_input: levels of an rdiv = 1 block */
+    int poison;         /* write NaN into the 2:1 ghost rows no sender fills */
+    int refined;        /* some block has rdiv > 1 */
+    int *ngrid;         /* [b] the block's own level count (dom(ib)%ngrid) */
     int itrsch;         /* mg_itrsch: 1 lexicographic GS, 2 ADI-TDMA, 3 red-black GS */
     double *ta, *tc;    /* TDMA recurrence arrays a(), c() of mgsolver.for:381 */
     orc_block *blk;
@@ -78,17 +100,72 @@ static inline size_t ci(const int32_t *n, int i, int j, int k) {
 #define DIMS(mg, b, l) ((mg)->dims[(size_t)(b) * (mg)->L + (l)])
 #define SLOT(mg, b, l) ((size_t)(b) * (mg)->L + (l))
 
+static int ilog2(int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
+}
+static int rdiv_of(const orc_block *b) { return b->rdiv > 0 ? b->rdiv : 1; }
+
+/* The grid level l of block b runs on (0-based: min(l, ngrid - 1)). */
+static int grid_level(const orc_mg *mg, int b, int l) {
+    return l < mg->ngrid[b] ? l : mg->ngrid[b] - 1;
+}
+/* Level l of block b repeats level l-1's grid (glevel > dom(ib)%ngrid). */
+static int repeats(const orc_mg *mg, int b, int l) { return l >= mg->ngrid[b]; }
+/* rdv(dom, glevel) of localparameters.for:40-54, glevel = l + 1. */
+static int rdv_at(const orc_mg *mg, int b, int l) {
+    if (l + 1 > mg->ngrid[b]) return 1;
+    if (l + 1 <= mg->lin) return rdiv_of(&mg->blk[b]);
+    return rdv_at(mg, b, l - 1) / 2;
+}
+
+/* Refined block sets this restatement covers (see the header). */
+static int refined_ok(int nb, const orc_block *blk) {
+    for (int b = 0; b < nb; ++b) {
+        int r = rdiv_of(&blk[b]);
+        if (r & (r - 1)) return 0;
+        for (int f = 0; f < 6; ++f) {
+            /* a periodic face only as a lone block's wrap onto itself */
+            if (blk[b].bc[f] == 5 && (blk[b].per[f] >= 0 || blk[b].nbr[f ^ 1] >= 0 || blk[b].nbr[f] >= 0))
+                return 0;
+            const int o = blk[b].nbr[f];
+            if (o < 0) continue;
+            const int ro = rdiv_of(&blk[o]);
+            if (ro != r && ro != 2 * r && 2 * ro != r) return 0;
+            for (int g = 0; g < 6; ++g) /* no edge neighbour (through two faces) */
+                if (g / 2 != f / 2 && blk[o].nbr[g] >= 0) return 0;
+        }
+    }
+    return 1;
+}
+
 orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
                    int nu_pre, int nu_post) {
     if (nblocks <= 0 || levels <= 0) return NULL;
-    for (int b = 0; b < nblocks; ++b)
+    int maxr = 1, refined = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        const int r = rdiv_of(&blocks[b]);
+        if (r > maxr) maxr = r;
+        if (r > 1) refined = 1;
+    }
+    if (refined && (lsm || !refined_ok(nblocks, blocks))) return NULL;
+    const int lin = levels - ilog2(maxr); /* ngrd_gl = ngrid_input + log2(max rdiv) */
+    if (lin < 1) return NULL;
+    for (int b = 0; b < nblocks; ++b) {
+        const int own = lin + ilog2(rdiv_of(&blocks[b]));
         for (int d = 0; d < 3; ++d) {
             int n = blocks[b].n[d];
-            if (n <= 0 || (n % (1 << (levels - 1))) != 0) return NULL;
+            if (n <= 0 || (n % (1 << (own - 1))) != 0) return NULL;
         }
+    }
     orc_mg *mg = (orc_mg *)calloc(1, sizeof(orc_mg));
     mg->nb = nblocks;
     mg->L = levels;
+    mg->lin = lin;
+    mg->refined = refined;
+    mg->ngrid = malloc(sizeof(int) * nblocks);
+    for (int b = 0; b < nblocks; ++b) mg->ngrid[b] = lin + ilog2(rdiv_of(&blocks[b]));
     mg->lsm = lsm;
     mg->nu_pre = nu_pre;
     mg->nu_post = nu_post;
@@ -111,7 +188,7 @@ orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
             if (blocks[b].bc[f] == 5) mg->periodic = 1; /* alloc_dom.for:1156-1163 */
         for (int l = 0; l < levels; ++l) {
             int32_t *n = DIMS(mg, b, l);
-            for (int d = 0; d < 3; ++d) n[d] = blocks[b].n[d] >> l;
+            for (int d = 0; d < 3; ++d) n[d] = blocks[b].n[d] >> grid_level(mg, b, l);
             mg->phi[SLOT(mg, b, l)] = calloc(gsz(n), sizeof(double));
             mg->rhs[SLOT(mg, b, l)] = calloc(isz(n), sizeof(double));
             mg->cof[SLOT(mg, b, l)] = calloc(7 * isz(n), sizeof(double));
@@ -154,8 +231,10 @@ void orc_destroy(orc_mg *mg) {
     }
     free(mg->phi); free(mg->rhs); free(mg->cof); free(mg->dens);
     free(mg->bufm); free(mg->bufp); free(mg->pf);
-    free(mg->dims); free(mg->blk); free(mg);
+    free(mg->dims); free(mg->blk); free(mg->ngrid); free(mg);
 }
+
+void orc_set_poison(orc_mg *mg, int on) { mg->poison = on != 0; }
 
 int orc_dims(const orc_mg *mg, int block, int level, int32_t out_n[3]) {
     if (block < 0 || block >= mg->nb || level < 0 || level >= mg->L) return -1;
@@ -214,9 +293,11 @@ void orc_coeff(orc_mg *mg) {
                             dl[ci(n, i, j, k)] = 0.125 * s;
                         }
             }
-            /* constant branch :184-189: 1/h_l^2 with h_l = h * 2^l */
-            double ndx = bk->h[0] * (double)(1 << l), ndy = bk->h[1] * (double)(1 << l),
-                   ndz = bk->h[2] * (double)(1 << l);
+            /* constant branch :184-189: 1/h_l^2 with h_l = h * 2^gl, gl the
+             * block's own grid level (:41-53) */
+            const int gl = grid_level(mg, b, l);
+            double ndx = bk->h[0] * (double)(1 << gl), ndy = bk->h[1] * (double)(1 << gl),
+                   ndz = bk->h[2] * (double)(1 << gl);
             double cxx = 1.0 / (ndx * ndx), cyy = 1.0 / (ndy * ndy), czz = 1.0 / (ndz * ndz);
             /* LSM branch :170-181 uses the finest dx, dy, dz on every level */
             double gx = bk->h[1] * bk->h[2] / bk->h[0];
@@ -341,24 +422,102 @@ static void bound_fields(orc_mg *mg, int l, double **f) {
     }
 }
 
-/* exchangepp (same-rank, same-resolution) followed by mgbound and
- * exchbc_mgpp; f[b] is block b's ghosted array at level l. */
+/* Ghosted index of the cell with normal coordinate c on axis d and
+ * tangential coordinates (t1, t2): (j, k) for d = 0, (i, k) for 1, (i, j)
+ * for 2 -- the plane_get order, which also indexes the staging buffers. */
+static inline size_t face_gi(const int32_t *n, int d, int c, int t1, int t2) {
+    return d == 0 ? gi(n, c, t1, t2) : (d == 1 ? gi(n, t1, c, t2) : gi(n, t1, t2, c));
+}
+static inline size_t face_m(const int32_t *n, int d, int t1, int t2) {
+    return (size_t)t1 + (size_t)(n[d == 0 ? 1 : 0] + 2) * (size_t)t2;
+}
+
+/* The 8 fine cells of a 2:1 fine-to-coarse ghost in the summation order of
+ * exchangepp.for:163-172 (x), :210-220 (y), :245-255 (z): offsets along
+ * (normal, t1, t2) from the low corner. */
+static const int avg8_order[3][8][3] = {
+    {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}}};
+
+/* Pack of block b (sender) for its face neighbour o on side hi (0: o is the
+ * prev neighbour, so b's first interior planes go out; 1: next) into buf,
+ * laid out on o's face. */
+static void pack_face(orc_mg *mg, int l, double **f, int b, int o, int d, int hi, double *buf) {
+    const int32_t *n = DIMS(mg, b, l), *no = DIMS(mg, o, l);
+    const int rb = rdv_at(mg, b, l), ro = rdv_at(mg, o, l);
+    if (rb == ro) { /* :151-161 */
+        plane_get(f[b], n, d, hi ? n[d] : 1, buf);
+        return;
+    }
+    const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+    if (rb > ro) { /* fine to coarse: 0.125 x 8 cells, o's interior rows */
+        const int c0 = hi ? n[d] - 1 : 1;
+        for (int t2 = 1; t2 <= no[a2]; ++t2)
+            for (int t1 = 1; t1 <= no[a1]; ++t1) {
+                double sum = 0.0;
+                for (int q = 0; q < 8; ++q) {
+                    const int *w = avg8_order[d][q];
+                    const double v = f[b][face_gi(n, d, c0 + w[0], 2 * t1 - 1 + w[1], 2 * t2 - 1 + w[2])];
+                    sum = q == 0 ? v : sum + v;
+                }
+                buf[face_m(no, d, t1, t2)] = 0.125 * sum;
+            }
+        return;
+    }
+    /* coarse to fine: 2 x the face cell to the 4 fine positions over it */
+    const int c = hi ? n[d] : 1;
+    for (int t2 = 1; t2 <= n[a2]; ++t2)
+        for (int t1 = 1; t1 <= n[a1]; ++t1) {
+            const double v = 2.0 * f[b][face_gi(n, d, c, t1, t2)];
+            for (int e2 = 0; e2 < 2; ++e2)
+                for (int e1 = 0; e1 < 2; ++e1) buf[face_m(no, d, 2 * t1 - 1 + e1, 2 * t2 - 1 + e2)] = v;
+        }
+}
+
+/* Unpack into block b's ghost plane on side hi from neighbour o's buf. */
+static void unpack_face(orc_mg *mg, int l, double **f, int b, int o, int d, int hi, const double *buf) {
+    const int32_t *n = DIMS(mg, b, l);
+    const int rb = rdv_at(mg, b, l), ro = rdv_at(mg, o, l);
+    const int g = hi ? n[d] + 1 : 0;
+    if (rb == ro) { /* :4038-4044 */
+        plane_set(f[b], n, d, g, buf);
+        return;
+    }
+    const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+    const int c1 = hi ? n[d] : 1, c2 = hi ? n[d] - 1 : 2;
+    for (int t2 = 0; t2 <= n[a2] + 1; ++t2)
+        for (int t1 = 0; t1 <= n[a1] + 1; ++t1) {
+            const int inner = t1 >= 1 && t1 <= n[a1] && t2 >= 1 && t2 <= n[a2];
+            const size_t q = face_gi(n, d, g, t1, t2);
+            if (!inner) { /* rows no sender fills */
+                if (rb > ro && mg->poison) f[b][q] = NAN;
+                continue;
+            }
+            const double r = buf[face_m(n, d, t1, t2)];
+            if (rb < ro) /* coarse side: :4054-4061 */
+                f[b][q] = r;
+            else /* fine side: :4046-4053 */
+                f[b][q] = ((r + 3.0 * f[b][face_gi(n, d, c1, t1, t2)]) - f[b][face_gi(n, d, c2, t1, t2)]) / 4.0;
+        }
+}
+
+/* exchangepp (same-rank branches) followed by mgbound and exchbc_mgpp;
+ * f[b] is block b's ghosted array at level l. */
 static void exchange_fields(orc_mg *mg, int l, double **f) {
     /* face passes x, y, z: all blocks pack, then all blocks unpack
-     * (exchangepp.for:43-430 pack, 3997-4127 unpack) */
+     * (exchangepp.for:43-430,1094-1240 pack, 3997-4345 unpack) */
     for (int d = 0; d < 3; ++d) {
         for (int b = 0; b < mg->nb; ++b) {
             const orc_block *bk = &mg->blk[b];
-            const int32_t *n = DIMS(mg, b, l);
             int prv = bk->nbr[2 * d], nxt = bk->nbr[2 * d + 1];
-            if (prv >= 0) plane_get(f[b], n, d, 1, mg->bufp[prv]);
-            if (nxt >= 0) plane_get(f[b], n, d, n[d], mg->bufm[nxt]);
+            if (prv >= 0) pack_face(mg, l, f, b, prv, d, 0, mg->bufp[prv]);
+            if (nxt >= 0) pack_face(mg, l, f, b, nxt, d, 1, mg->bufm[nxt]);
         }
         for (int b = 0; b < mg->nb; ++b) {
             const orc_block *bk = &mg->blk[b];
-            const int32_t *n = DIMS(mg, b, l);
-            if (bk->nbr[2 * d] >= 0) plane_set(f[b], n, d, 0, mg->bufm[b]);
-            if (bk->nbr[2 * d + 1] >= 0) plane_set(f[b], n, d, n[d] + 1, mg->bufp[b]);
+            if (bk->nbr[2 * d] >= 0) unpack_face(mg, l, f, b, bk->nbr[2 * d], d, 0, mg->bufm[b]);
+            if (bk->nbr[2 * d + 1] >= 0) unpack_face(mg, l, f, b, bk->nbr[2 * d + 1], d, 1, mg->bufp[b]);
         }
     }
     bound_fields(mg, l, f); /* exchangepp.for:4761 */
@@ -608,6 +767,11 @@ void orc_restrict(orc_mg *mg, int lc) {
         memset(rc, 0, isz(nc) * sizeof(double)); /* :659-667 */
         double *resf = malloc(isz(nf) * sizeof(double));
         residual_block(mg, b, lf, mg->phi[SLOT(mg, b, lf)], mg->rhs[SLOT(mg, b, lf)], resf);
+        if (repeats(mg, b, lc)) { /* injection on the repeated grid (:719-732) */
+            memcpy(rc, resf, isz(nc) * sizeof(double));
+            free(resf);
+            continue;
+        }
 #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nc[2]; ++k)
             for (int j = 1; j <= nc[1]; ++j)
@@ -639,6 +803,10 @@ void orc_correct(orc_mg *mg, int lf) {
         const int32_t *nc = DIMS(mg, b, lc), *nf = DIMS(mg, b, lf);
         const double *pc = mg->phi[SLOT(mg, b, lc)];
         double *pfi = mg->phi[SLOT(mg, b, lf)];
+        if (repeats(mg, b, lc)) { /* injection, ghosts included (:807-819) */
+            for (size_t g = 0; g < gsz(nf); ++g) pfi[g] = pfi[g] + pc[g];
+            continue;
+        }
         /* coarse plane kc writes fine planes 2kc and 2kc+1 only */
 #pragma omp parallel for schedule(static)
         for (int kc = 0; kc <= nc[2]; ++kc)

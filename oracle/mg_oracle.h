@@ -25,11 +25,17 @@ typedef struct {
     int32_t nbr[6];   /* face neighbour block index, -1 if none */
     int32_t per[6];   /* periodic partner block index (per_ip..per_kn), -1 if none */
     int32_t bc[6];    /* boundary codes; 5 = periodic, anything else Neumann for p */
+    int32_t rdiv;     /* refinement ratio rdiv (infodom.cin), a power of 2; 0 = 1.
+                         n is at the block's own resolution, h its own spacing */
 } orc_block;
 
 typedef struct orc_mg orc_mg;
 
-/* lsm: 0 constant coefficients; 1 the reference's density hierarchy
+/* levels: ngrd_gl = ngrid_input + log2(max rdiv).  Returns NULL for a block
+ * set outside the covered scope (extents not divisible by 2^(own levels - 1);
+ * refined blocks with lsm, a periodic face, face neighbours more than a
+ * factor 2 apart in rdiv, or an edge neighbour).
+ * lsm: 0 constant coefficients; 1 the reference's density hierarchy
  * (mgsolver.for:69-181); 2 the same finest level with coarse operators by
  * face-coefficient restriction (the library's MF3D_COARSE_FACE). */
 orc_mg *orc_create(int nblocks, const orc_block *blocks, int levels, int lsm,
@@ -56,6 +62,11 @@ int     orc_solve(orc_mg *mg, const double *f, double *p, double tol, int maxcy,
  * w = A d, alpha = (r, w)/(w, w), p += alpha d, r = f - A p. */
 int     orc_solve_gcr(orc_mg *mg, const double *f, double *p, double tol, int maxcy,
                       int min_cycles, double *hist_rho, double *hist_rmax);
+
+/* Refined blocks: write NaN into the ghost rows of a 2:1 fine-side ghost
+ * face that no sender fills (the reference rewrites them from stale receive
+ * buffer entries), so a result that depends on them shows NaN. */
+void    orc_set_poison(orc_mg *mg, int on);
 
 /* Unit-level access for kernel tests. */
 int     orc_dims(const orc_mg *mg, int block, int level, int32_t out_n[3]);

@@ -25,7 +25,7 @@ REF_VEL_BIN = os.path.join(HERE, "_ref", "mg_ref_vel")
 class OrcBlock(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32 * 3), ("h", ctypes.c_double * 3),
                 ("nbr", ctypes.c_int32 * 6), ("per", ctypes.c_int32 * 6),
-                ("bc", ctypes.c_int32 * 6)]
+                ("bc", ctypes.c_int32 * 6), ("rdiv", ctypes.c_int32)]
 
 
 _libs = {}
@@ -46,6 +46,7 @@ def lib(omp: bool = False):
         L.orc_set_density.argtypes = [vp, dp]
         L.orc_set_smoother.restype = ip
         L.orc_set_smoother.argtypes = [vp, ip]
+        L.orc_set_poison.argtypes = [vp, ip]
         L.orc_coeff.argtypes = [vp]
         L.orc_vcycle.argtypes = [vp, dp, dp]
         L.orc_solve.restype = ip
@@ -96,10 +97,12 @@ class OracleMG:
             arr[i].nbr[:] = list(b.nbr)
             arr[i].per[:] = list(b.per)
             arr[i].bc[:] = list(b.bc)
+            arr[i].rdiv = int(getattr(b, "rdiv", 1))
         mode = (2 if coarse_op == "face" else 1) if lsm else 0
         self._h = self._L.orc_create(len(self.blocks), arr, levels, mode, nu_pre, nu_post)
         if not self._h:
-            raise ValueError("oracle rejected the block set (extent not divisible by 2**(L-1)?)")
+            raise ValueError("oracle rejected the block set (extent not divisible by 2**(levels-1), "
+                             "or refined blocks outside the covered scope)")
         if self._L.orc_set_smoother(self._h, smoother) != 0:
             raise ValueError(f"unknown smoother {smoother}")
 
@@ -109,6 +112,10 @@ class OracleMG:
             self._h = None
 
     __del__ = close
+
+    def set_poison(self, on: bool = True):
+        """NaN into the 2:1 fine-side ghost rows no sender fills."""
+        self._L.orc_set_poison(self._h, int(on))
 
     def dims(self, block: int, level: int):
         out = (ctypes.c_int32 * 3)()
@@ -228,7 +235,9 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
         ctl = os.path.join(td, "ctl.txt")
         with open(ctl, "w") as fh:
             nbx, nby, nbz = grid.shape
-            fh.write(f"{nbx} {nby} {nbz} {levels} {int(lsm)} {maxcy} {min_cycles} {nu_pre} {nu_post}\n")
+            # levels is ngrd_gl; the driver takes ngrid_input = ngrd_gl - log2(max rdiv)
+            lin = levels - (max(getattr(b, "rdiv", 1) for b in grid.blocks).bit_length() - 1)
+            fh.write(f"{nbx} {nby} {nbz} {lin} {int(lsm)} {maxcy} {min_cycles} {nu_pre} {nu_post}\n")
             fh.write(f"{tol!r} {smoother}\n")
             fh.write(" ".join(str(c) for c in grid.bc) + "\n")
             fh.write(" ".join(map(str, grid.ex)) + "\n")
@@ -236,6 +245,8 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
             fh.write(" ".join(map(str, grid.ez)) + "\n")
             fh.write(" ".join(repr(float(x)) for x in grid.h) + "\n")
             fh.write(frhs + "\n" + fden + "\n" + fout + "\n")
+            if any(getattr(b, "rdiv", 1) > 1 for b in grid.blocks):
+                fh.write(" ".join(str(b.rdiv) for b in grid.blocks) + "\n")
         subprocess.run([REF_BIN, ctl], check=True, capture_output=True, timeout=600)
         raw = np.fromfile(fout, dtype=np.float64)
     k = int(raw[0])

@@ -18,6 +18,8 @@
 !   line 8: rhs file  (f64, per block interior, i fastest; blocks x fastest)
 !   line 9: density file or '-'   (same layout)
 !   line 10: output file
+!   line 11 (optional): rdiv of every block (blocks x fastest); a block's
+!            extents are its column's extents x rdiv, its spacing dx / rdiv
 ! Output (f64 stream): ncycles, rho(0:maxcy), rmax(0:maxcy), then per block
 ! the interior of p, then per block pp after cycle 1 including one ghost
 ! layer.
@@ -36,7 +38,8 @@ program mg_ref_driver
   double precision, allocatable :: rho(:), rmx(:)
   character(len=1024) :: ctl, frhs, fdens, fout
   type(fieldp), allocatable :: f(:), pp1(:)
-  integer :: mi, mj, mk, nwork, pre, tot, u
+  integer :: mi, mj, mk, nwork, pre, tot, u, ios, maxr, gl, r
+  integer, allocatable :: rdivs(:)
 
   call get_command_argument(1, ctl)
   open(newunit=u, file=trim(ctl), status='old')
@@ -51,18 +54,26 @@ program mg_ref_driver
   read(u, '(a)') frhs
   read(u, '(a)') fdens
   read(u, '(a)') fout
+  num_domains = nbx * nby * nbz
+  allocate(rdivs(num_domains))
+  read(u, *, iostat=ios) rdivs
+  if (ios /= 0) rdivs = 1
   close(u)
 
-  num_domains = nbx * nby * nbz
   nbp = num_domains
-  ngrd_gl = ngrid_input
+  ! global level count and per-level ratios as localparameters.for:26-54
+  ! computes them (ngrd_gl = ngrid_input + log2(max rdiv))
+  maxr = maxval(rdivs)
+  ngrd_gl = ngrid_input + nint(log(real(maxr)) / log(2.0))
   L_LSM = (lsmf == 1)
   pl = 1
   g_dx = gdx; g_dy = gdy; g_dz = gdz
   allocate(dom(nbp), dom_id(nbp), dom_ad(0:num_domains-1), dom_indid(0:num_domains-1))
   allocate(rdiv(0:num_domains-1), rdv(0:num_domains-1, ngrd_gl))
-  rdiv = 1
-  rdv = 1
+  do id = 0, num_domains - 1
+    rdiv(id) = rdivs(id + 1)
+  end do
+  rdv = 0
   PERIODIC = any(bcs == 5)
   ngg = 0
 
@@ -74,13 +85,24 @@ program mg_ref_driver
     dom_id(ib) = id
     dom_ad(id) = 0
     dom_indid(id) = ib
-    nn = (/ ex(ia+1), ey(ib2+1), ez(ic2+1) /)
+    r = rdiv(id)
+    nn = (/ ex(ia+1), ey(ib2+1), ez(ic2+1) /) * r
     dom(ib)%isp = 1 + pl; dom(ib)%iep = nn(1) + pl
     dom(ib)%jsp = 1 + pl; dom(ib)%jep = nn(2) + pl
     dom(ib)%ksp = 1 + pl; dom(ib)%kep = nn(3) + pl
     dom(ib)%ttc_i = nn(1) + 2*pl; dom(ib)%ttc_j = nn(2) + 2*pl; dom(ib)%ttc_k = nn(3) + 2*pl
-    dom(ib)%ngrid = ngrd_gl
-    dom(ib)%dx = gdx; dom(ib)%dy = gdy; dom(ib)%dz = gdz
+    dom(ib)%ngrid = ngrid_input + nint(log(real(r)) / log(2.0))
+    do l = 1, ngrd_gl
+      if (l > dom(ib)%ngrid) then
+        rdv(id, l) = 1
+      else if (l <= ngrid_input) then
+        rdv(id, l) = r
+      else
+        rdv(id, l) = rdv(id, l-1) / 2
+      end if
+    end do
+    ! localparameters.for:255-257
+    dom(ib)%dx = gdx / real(r); dom(ib)%dy = gdy / real(r); dom(ib)%dz = gdz / real(r)
     dom(ib)%bc_west = bcs(1); dom(ib)%bc_east = bcs(2)
     dom(ib)%bc_south = bcs(3); dom(ib)%bc_north = bcs(4)
     dom(ib)%bc_bottom = bcs(5); dom(ib)%bc_top = bcs(6)
@@ -114,7 +136,7 @@ program mg_ref_driver
   end do
   end do
   end do
-  nge = maxval((/ maxval(ex), maxval(ey), maxval(ez) /) + 2*pl) * (pl+1)**2
+  nge = (maxval((/ maxval(ex), maxval(ey), maxval(ez) /)) * maxr + 2*pl) * (pl+1)**2
   ngc = (pl+1)**3
 
   allocate(f(nbp), pp1(nbp))
@@ -128,6 +150,9 @@ program mg_ref_driver
              dom(ib)%recvb_p1(ngg), dom(ib)%sendb_m2(ngg), dom(ib)%sendb_p2(ngg), &
              dom(ib)%recvb_m2(ngg), dom(ib)%recvb_p2(ngg), dom(ib)%sendb_m3(ngg), &
              dom(ib)%sendb_p3(ngg), dom(ib)%recvb_m3(ngg), dom(ib)%recvb_p3(ngg))
+    ! defined start for the receive buffers (a 2:1 fine side reads entries no
+    ! sender fills)
+    dom(ib)%recvb_m1 = 0.0; dom(ib)%recvb_p1 = 0.0
     allocate(dom(ib)%sc1m(ngc), dom(ib)%sc1p(ngc), dom(ib)%rc1m(ngc), dom(ib)%rc1p(ngc), &
              dom(ib)%sc2m(ngc), dom(ib)%sc2p(ngc), dom(ib)%rc2m(ngc), dom(ib)%rc2p(ngc), &
              dom(ib)%sc3m(ngc), dom(ib)%sc3p(ngc), dom(ib)%rc3m(ngc), dom(ib)%rc3p(ngc), &
@@ -142,9 +167,10 @@ program mg_ref_driver
     allocate(dom(ib)%faz(ngrd_gl))
     nwork = 0
     do l = 1, ngrd_gl
-      mi = (dom(ib)%iep-dom(ib)%isp+1)/2**(l-1)+2
-      mj = (dom(ib)%jep-dom(ib)%jsp+1)/2**(l-1)+2
-      mk = (dom(ib)%kep-dom(ib)%ksp+1)/2**(l-1)+2
+      gl = min(l, dom(ib)%ngrid)
+      mi = (dom(ib)%iep-dom(ib)%isp+1)/2**(gl-1)+2
+      mj = (dom(ib)%jep-dom(ib)%jsp+1)/2**(gl-1)+2
+      mk = (dom(ib)%kep-dom(ib)%ksp+1)/2**(gl-1)+2
       nwork = nwork + mi*mj*mk + 8*(mi-2)*(mj-2)*(mk-2)
     end do
     allocate(dom(ib)%cof(nwork))

@@ -1,0 +1,70 @@
+"""Write golden vectors for refined block grids with the compiled reference
+(oracle/_ref/mg_ref, which takes each block's rdiv).
+
+Runs in the build container only (the reference sources exist there):
+
+    make -C oracle ref && python tests/golden/make_golden_refined.py
+
+Each case is a row of blocks along one axis, some refined (rdiv 2 or 4:
+extents x rdiv, spacing / rdiv), with walls or a block's own periodic wrap
+on the other axes.  The reference's coeff + mgkcyc + exchangepp (its 2:1
+branches: 8-cell averages to the coarse side, (2 phi_c + 3 phi_1 - phi_2)/4
+on the fine side, injection on the repeated coarsest grid) run in the
+Poisson form of the newsolv_mg loop for a fixed number of cycles on a
+seeded RHS.  <case>.npz holds f, rho, rmax, p and pp1 as make_golden.py
+writes them; manifest_refined.json records how each case was made
+(levels = ngrd_gl, the global level count).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from multiflow3d_amd import topology as T  # noqa: E402
+from multiflow3d_amd import workloads as W  # noqa: E402
+from oracle.oracle import run_ref  # noqa: E402
+
+N, P = T.NEUMANN, T.PERIODIC
+WALLS = [N] * 6
+# name: (ex, ey, ez, bc, h, rdiv, levels, cycles, seed)
+CASES = {
+    "ref_x_cf_walls_L4": ([8, 8], [8], [8], WALLS, [1 / 16] * 3, [1, 2], 4, 6, W.BASE_SEED + 61),
+    "ref_x_fc_zper_L4": ([8, 8], [8], [8], [N, N, N, N, P, P], [1 / 16] * 3, [2, 1], 4, 6, W.BASE_SEED + 62),
+    "ref_y_cf_xper_L3": ([8], [8, 12], [8], [P, P, N, N, N, N], [1 / 16, 1 / 20, 1 / 8], [1, 2], 3, 6,
+                         W.BASE_SEED + 63),
+    "ref_z_fc_walls_L4": ([8], [8], [8, 8], WALLS, [1 / 8, 1 / 8, 1 / 16], [2, 1], 4, 6, W.BASE_SEED + 64),
+    "ref_x_cfc_yper_L3": ([8, 12, 8], [8], [8], [N, N, P, P, N, N], [1 / 28, 1 / 8, 1 / 8], [1, 2, 1], 3, 6,
+                          W.BASE_SEED + 65),
+    "ref_x_r2r4_L4": ([8, 8], [8], [8], WALLS, [1 / 16] * 3, [2, 4], 4, 5, W.BASE_SEED + 66),
+}
+
+
+def grid_of(spec):
+    ex, ey, ez, bc, h, rdiv = spec[:6]
+    return T.refined_grid(ex, ey, ez, bc, h, rdiv)
+
+
+def main():
+    manifest = {}
+    for name, spec in CASES.items():
+        ex, ey, ez, bc, h, rdiv, L, cyc, seed = spec
+        g = grid_of(spec)
+        f = W.block_rhs(g, seed)
+        p, rho, rmax, k, pp1 = run_ref(g, L, f, 0.0, cyc)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), f=np.concatenate([a.ravel() for a in f]),
+                            rho=rho, rmax=rmax, p=np.concatenate([a.ravel() for a in p]),
+                            pp1=np.concatenate([a.ravel() for a in pp1]))
+        manifest[name] = dict(ex=ex, ey=ey, ez=ez, bc=bc, h=h, rdiv=rdiv, levels=L, cycles=cyc, seed=seed,
+                              tol=0.0)
+        print(name, "cycles", k, "rho_last", rho[-1], flush=True)
+    with open(os.path.join(HERE, "manifest_refined.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()
