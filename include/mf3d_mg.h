@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MF3D_MG_ABI_VERSION 6
+#define MF3D_MG_ABI_VERSION 7
 
 enum {
     MF3D_OK = 0,
@@ -103,6 +103,10 @@ typedef struct {
     int32_t nbr[6];  /* iprev, inext, jprev, jnext, kprev, knext: block index or -1 */
     int32_t per[6];  /* per_ip, per_in, per_jp, per_jn, per_kp, per_kn: block index or -1 */
     int32_t bc[6];   /* bc_west .. bc_top; 5 = periodic, any other code = Neumann for p */
+    int32_t rdiv;    /* rdiv(dom_id(ib)) (infodom.cin, localparameters.for:26-56): a power of
+                        2, 0 = 1.  n and h are the block's own (refined) extents and
+                        spacing; the block runs levels - log2(max rdiv) + log2(rdiv) levels
+                        of its own and repeats its coarsest grid beyond them */
 } mf3d_block_desc;
 
 typedef struct {
@@ -280,6 +284,21 @@ int64_t mf3d_mg_device_bytes(const mf3d_mg *mg);
 
 /* Bytes a handle with this configuration would hold, without creating it. */
 int64_t mf3d_mg_estimate_bytes(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks);
+
+/* The ghost fill of one level -- exchangepp (with its 2:1 branches on
+ * refined grids), mgbound and exchbc_mgpp -- as a table of expressions over
+ * interior cells, the form the library applies on refined block grids
+ * (no counterpart in the reference; for host-side checks of the exchange).
+ * Host only, no device work.  Entries are 10 int32 each: destination, kind
+ * (0 copy of src0; 1 0.125 x (src0 + ... + src7) summed in that order;
+ * 2 ((2 src0 + 3 src1) - src2) / 4), src0 .. src7, as element offsets into
+ * the level's arena (every block's (n0+2)(n1+2)(n2+2) array, i fastest,
+ * concatenated in block order).  Writes at most cap entries to out (may be
+ * NULL when cap is 0) and the table length to *count; MF3D_EINVAL /
+ * MF3D_ENOTSUP as mf3d_mg_create would return for the configuration, with
+ * the reason in err (err_len bytes, may be NULL). */
+int mf3d_mg_ghost_table(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, int32_t level, int32_t *out,
+                        int64_t cap, int64_t *count, char *err, size_t err_len);
 
 /* Measurement hook (no counterpart in the reference; for benchmarks):
  * while enabled, every finest-level launch of the chosen kernel is

@@ -25,6 +25,7 @@ EXPORTS = (
     "mf3d_mg_ghosted_cells", "mf3d_mg_set_density_device", "mf3d_mg_heaviside",
     "mf3d_mg_heaviside_device", "mf3d_tecbin_header", "mf3d_tecbin_read", "mf3d_final_ctime_read",
     "mf3d_io_last_error", "mf3d_mg_solve_device_noalloc", "mf3d_mg_shell_cells",
+    "mf3d_mg_ghost_table",
 )
 MF3D_NOALLOC_SCRATCH_BYTES = 32 << 20
 MF3D_ENOENT, MF3D_EFORMAT = -2, -74
@@ -39,7 +40,7 @@ class BlockDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32 * 3), ("ghost", ctypes.c_int32),
                 ("ld", ctypes.c_int32 * 3), ("h", ctypes.c_double * 3),
                 ("nbr", ctypes.c_int32 * 6), ("per", ctypes.c_int32 * 6),
-                ("bc", ctypes.c_int32 * 6)]
+                ("bc", ctypes.c_int32 * 6), ("rdiv", ctypes.c_int32)]
 
 
 class TecbinHdr(ctypes.Structure):
@@ -122,6 +123,10 @@ def lib():
     L.mf3d_mg_ghosted_cells.argtypes = [vp]
     L.mf3d_mg_shell_cells.restype = ctypes.c_int64
     L.mf3d_mg_shell_cells.argtypes = [vp]
+    L.mf3d_mg_ghost_table.restype = ctypes.c_int
+    L.mf3d_mg_ghost_table.argtypes = [ctypes.POINTER(Cfg), ctypes.POINTER(BlockDesc), ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_size_t]
     L.mf3d_mg_set_density_device.restype = ctypes.c_int
     L.mf3d_mg_set_density_device.argtypes = [vp, vp, vp]
     L.mf3d_mg_heaviside.restype = ctypes.c_int

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmf3d_mg.so")
 SRC = [os.path.join(HERE, "csrc", "mf3d_mg.hip")]
-DEPS = SRC + [os.path.join(HERE, "csrc", h) for h in ("mg_common.hpp", "mg_outer.hpp", "mg_setup.hpp", "mg_sweep.hpp", "mg_tail.hpp", "mg_levelset.hpp", "mg_linesmooth.hpp", "mf3d_tecbin.hpp", "mg_transfer.hpp", "mg_velocity.hpp", "mg_coarse.hpp", "mg_ghostmap.hpp")] + [os.path.join(ROOT, "include", "mf3d_mg.h")]
+DEPS = SRC + [os.path.join(HERE, "csrc", h) for h in ("mg_common.hpp", "mg_outer.hpp", "mg_setup.hpp", "mg_sweep.hpp", "mg_tail.hpp", "mg_levelset.hpp", "mg_linesmooth.hpp", "mf3d_tecbin.hpp", "mg_transfer.hpp", "mg_velocity.hpp", "mg_coarse.hpp", "mg_ghostmap.hpp", "mg_refine.hpp")] + [os.path.join(ROOT, "include", "mf3d_mg.h")]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MF3D_OFFLOAD_ARCH", "gfx950")

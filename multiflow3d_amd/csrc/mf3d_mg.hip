@@ -35,6 +35,7 @@
 #include "mg_coarse.hpp"
 #include "mg_common.hpp"
 #include "mg_ghostmap.hpp"
+#include "mg_refine.hpp"
 #include "mg_outer.hpp"
 #include "mg_setup.hpp"
 #include "mg_sweep.hpp"
@@ -72,6 +73,18 @@ struct Level {
     int sw_ty = 4;  // sweep tile rows of the march; 0: 3-D tiles (k_rb3d)
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
+    // refined block grids: the level's ghosts as expressions (no regions)
+    GhostExpr *gx = nullptr;
+    int n_gx = 0;
+    // restriction into this level: k_restrict tiles over blocks that coarsen
+    // here, k_restrict_rep tiles over blocks that repeat their grid here
+    Tile *t_rst = nullptr, *t_rrep = nullptr;
+    int n_rst = 0, n_rrep = 0;
+    // prolongation into this level from the next: k_prolong tiles (t_oct)
+    // exclude blocks that repeat their grid at the next level, whose rows
+    // k_prolong_rep takes
+    Tile *t_prep = nullptr;
+    int n_prep = 0;
     bool own_ghosts = false;  // lone block: the sweep writes the ghosts, no gather
     long long cells_gh = 0;  // arena length (all blocks, ghosted)
     void *phi[2] = {nullptr, nullptr};
@@ -87,6 +100,8 @@ struct Level {
 struct mf3d_mg {
     mf3d_mg_cfg cfg{};
     std::vector<mf3d_block_desc> blk;
+    std::vector<int> own;  // each block's own level count (refined blocks have more)
+    bool refined = false;  // some block has rdiv > 1
     hipStream_t stream = nullptr;
     std::vector<Level> lv;
     double *pbuf[2] = {nullptr, nullptr};
@@ -244,9 +259,94 @@ int upload(mf3d_mg *mg, X **dst, const std::vector<X> &src) {
     return 0;
 }
 
-inline int dim_at(const mf3d_block_desc &b, int d, int l) { return b.n[d] >> l; }
+inline int rdiv_of(const mf3d_block_desc &b) { return b.rdiv > 0 ? b.rdiv : 1; }
+inline int lg2i(int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
+}
+
+// Each block's own level count (localparameters.for:33-56): levels is
+// ngrd_gl = ngrid_input + log2(max rdiv), and a block of rdiv r runs
+// ngrid_input + log2(r) levels of its own.
+std::vector<int> own_levels(int nb, const mf3d_block_desc *b, int L) {
+    int top = 0;
+    for (int i = 0; i < nb; ++i) top = std::max(top, lg2i(rdiv_of(b[i])));
+    std::vector<int> own(nb);
+    for (int i = 0; i < nb; ++i) own[i] = L - top + lg2i(rdiv_of(b[i]));
+    return own;
+}
+
+// Extent along d at level l of a block with `own` levels: beyond them it
+// repeats its coarsest grid (mgsolver.for:26-34, gl = min(glevel, ngrid)).
+inline int dim_at(const mf3d_block_desc &b, int d, int l, int own = 1 << 30) {
+    return b.n[d] >> std::min(l, std::max(own, 1) - 1);
+}
+
+// rdv(dom, glevel) of localparameters.for:40-54 at level l (glevel l + 1).
+int rdv_at(const mf3d_block_desc &b, int own, int lin, int l) {
+    if (l + 1 > own) return 1;
+    if (l + 1 <= lin) return rdiv_of(b);
+    return rdv_at(b, own, lin, l - 1) / 2;
+}
 
 bool is_neumann(const mf3d_block_desc &b, int f) { return b.nbr[f] < 0 && b.bc[f] != 5; }
+
+bool any_refined(int nb, const mf3d_block_desc *b) {
+    for (int i = 0; i < nb; ++i)
+        if (rdiv_of(b[i]) > 1) return true;
+    return false;
+}
+
+// Ghost table of level l of a refined block grid (mg_refine.hpp): "" or why
+// the level is refused.
+std::string ghost_exprs(const std::vector<mf3d_block_desc> &blk, const std::vector<int> &own, int L, int l,
+                        std::vector<GhostExpr> *out) {
+    int top = 0;
+    for (const mf3d_block_desc &B : blk) top = std::max(top, lg2i(rdiv_of(B)));
+    std::vector<SymBlock> sb(blk.size());
+    long long off = 0;
+    for (size_t b = 0; b < blk.size(); ++b) {
+        for (int d = 0; d < 3; ++d) sb[b].n[d] = dim_at(blk[b], d, l, own[b]);
+        sb[b].off = off;
+        sb[b].rdv = rdv_at(blk[b], own[b], L - top, l);
+        sb[b].desc = &blk[b];
+        off += (long long)(sb[b].n[0] + 2) * (sb[b].n[1] + 2) * (sb[b].n[2] + 2);
+    }
+    GhostSym sym(sb);
+    sym.run();
+    return sym.emit(out);
+}
+
+// What refined block grids this library runs: the exchange's 2:1 branches
+// are face passes only (no block may touch another along an edge or at a
+// corner alone: the reference's corner and edge messages have 2:1 branches
+// of their own, exchangepp.for:433-1093), a periodic axis only as a lone
+// block's wrap onto itself (exchbc_mgpp has no 2:1 branch), constant
+// coefficients, red-black smoothing, the defect-correction Poisson form.
+std::string refined_scope(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
+    if (!any_refined(c.nblocks, b)) return "";
+    if (c.variable_coef) return "refined blocks (rdiv > 1) with variable coefficients";
+    if (c.smoother != 3) return "refined blocks (rdiv > 1) with a smoother other than red-black (mg_itrsch 3)";
+    if (c.krylov != MF3D_OUTER_DEFECT) return "refined blocks (rdiv > 1) with the GCR outer iteration";
+    if (c.coarse_solve == MF3D_COARSEST_DIRECT) return "refined blocks (rdiv > 1) with the direct coarsest solve";
+    for (int i = 0; i < c.nblocks; ++i)
+        for (int f = 0; f < 6; ++f) {
+            if (b[i].bc[f] == 5 && (b[i].per[f] >= 0 || b[i].nbr[f] >= 0 || b[i].nbr[f ^ 1] >= 0))
+                return "refined blocks (rdiv > 1): a periodic axis only where a lone block wraps onto itself";
+            const int o = b[i].nbr[f];
+            if (o < 0) continue;
+            const int ri = rdiv_of(b[i]), ro = rdiv_of(b[o]);
+            if (ri != ro && ri != 2 * ro && ro != 2 * ri)
+                return "refined blocks (rdiv > 1): face neighbours " + std::to_string(i) + " and " +
+                       std::to_string(o) + " differ by more than a factor 2";
+            for (int g = 0; g < 6; ++g)
+                if (g / 2 != f / 2 && b[o].nbr[g] >= 0)
+                    return "refined blocks (rdiv > 1): blocks that touch along an edge (block " + std::to_string(i) +
+                           " and " + std::to_string(b[o].nbr[g]) + ")";
+        }
+    return "";
+}
 
 // Validate the block set: extents divisible by 2^(L-1), neighbours in range,
 // symmetric and conforming (equal extents across a shared face).
@@ -265,8 +365,14 @@ std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (c.differencing == 2 && c.variable_coef)
         return "differencing = 2 with a variable coefficient: the reference runs its level set with WENO "
                "differencing only (initial.for:133-135)";
-    const int div = 1 << (c.levels - 1);
+    const std::vector<int> own = own_levels(c.nblocks, b, c.levels);
     for (int i = 0; i < c.nblocks; ++i) {
+        if (b[i].rdiv < 0 || (b[i].rdiv & (b[i].rdiv - 1)))
+            return "block " + std::to_string(i) + ": rdiv must be a power of 2";
+        if (own[i] < 1) return "levels must exceed log2 of the largest rdiv";
+    }
+    for (int i = 0; i < c.nblocks; ++i) {
+        const int div = 1 << (own[i] - 1);
         for (int d = 0; d < 3; ++d) {
             if (b[i].n[d] <= 0) return "block " + std::to_string(i) + ": empty extent";
             if (b[i].n[d] % div)
@@ -288,8 +394,8 @@ std::string validate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
                 if (kind == 1 && b[i].bc[f] != 5) return "periodic partner on a non-periodic face";
                 const int back = kind == 0 ? b[nb].nbr[opp] : b[nb].per[opp];
                 if (back != i) return "neighbour table not symmetric at block " + std::to_string(i);
-                for (int e = 0; e < 3; ++e)
-                    if (e != d && b[nb].n[e] != b[i].n[e])
+                for (int e = 0; e < 3; ++e)  // equal physical extent: n x (rdiv of the other)
+                    if (e != d && (long long)b[nb].n[e] * rdiv_of(b[i]) != (long long)b[i].n[e] * rdiv_of(b[nb]))
                         return "blocks " + std::to_string(i) + " and " + std::to_string(nb) +
                                " do not conform on their shared face";
             }
@@ -448,12 +554,13 @@ std::vector<Tile> xcd_order(const std::vector<Tile> &t) {
 inline int inner_bytes(const mf3d_mg_cfg &c) { return c.precision == MF3D_MIXED ? 4 : 8; }
 
 long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
+    const std::vector<int> own = own_levels(c.nblocks, b, c.levels);
     long long tot = 0;
     const long long w = inner_bytes(c);
     for (int l = 0; l < c.levels; ++l) {
         long long gh = 0;
         for (int i = 0; i < c.nblocks; ++i)
-            gh += (long long)(dim_at(b[i], 0, l) + 2) * (dim_at(b[i], 1, l) + 2) * (dim_at(b[i], 2, l) + 2);
+            gh += (long long)(dim_at(b[i], 0, l, own[i]) + 2) * (dim_at(b[i], 1, l, own[i]) + 2) * (dim_at(b[i], 2, l, own[i]) + 2);
         tot += gh * w * 3;                                     // phi (two) and rhs
         if (c.variable_coef) tot += gh * (w * 3 + 8 * 4);      // coefficients, FP64 copies, density
         if (c.smoother == 2) tot += gh * w * 6;                // ADI Thomas coefficients (LDS levels)
@@ -471,16 +578,16 @@ long long estimate(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (c.levels >= 2 && c.precision == MF3D_MIXED) {  // FP64 coarsest potential (two) and RHS
         long long ghc = 0;
         for (int i = 0; i < c.nblocks; ++i)
-            ghc += (long long)(dim_at(b[i], 0, c.levels - 1) + 2) * (dim_at(b[i], 1, c.levels - 1) + 2) *
-                   (dim_at(b[i], 2, c.levels - 1) + 2);
+            ghc += (long long)(dim_at(b[i], 0, c.levels - 1, own[i]) + 2) * (dim_at(b[i], 1, c.levels - 1, own[i]) + 2) *
+                   (dim_at(b[i], 2, c.levels - 1, own[i]) + 2);
         tot += 3 * 8 * ghc;
     }
     const bool direct = c.levels >= 2 && c.coarse_solve == MF3D_COARSEST_DIRECT;
     if (direct) {  // the coarsest level's inverse operator
         long long nc = 0;
         for (int i = 0; i < c.nblocks; ++i)
-            nc += (long long)dim_at(b[i], 0, c.levels - 1) * dim_at(b[i], 1, c.levels - 1) *
-                  dim_at(b[i], 2, c.levels - 1);
+            nc += (long long)dim_at(b[i], 0, c.levels - 1, own[i]) * dim_at(b[i], 1, c.levels - 1, own[i]) *
+                  dim_at(b[i], 2, c.levels - 1, own[i]);
         if (nc <= MF3D_COARSEST_MAX_CELLS) tot += nc * nc * 8 + nc * 64;
     }
     return tot;
@@ -546,13 +653,25 @@ void fill(mf3d_mg *mg, T *a, long long n, const int *done = nullptr) {
     hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, mg->stream, a, n, T(0), done);
 }
 
+// Every ghost of a level-l arena a (laid out as the level) from its
+// interior: the region table, or a refined grid's expression table.
+template <typename T>
+void fill_ghosts(mf3d_mg *mg, const Level &L, T *a, const int *done) {
+    if (L.n_gx) {
+        const unsigned gb = (unsigned)std::min(4096, (L.n_gx + 255) / 256);
+        hipLaunchKernelGGL(k_gather_expr<T>, dim3(gb), dim3(256), 0, mg->stream, a, L.gx, L.n_gx, done);
+        return;
+    }
+    if (!L.n_full) return;
+    const int gx = std::max(1, std::min(64, (L.max_full + 255) / 256));
+    hipLaunchKernelGGL(k_gather<T>, dim3(gx, L.n_full), dim3(256), 0, mg->stream, a, L.db, L.r_full, done);
+}
+
 template <typename T>
 void gather(mf3d_mg *mg, int l, const int *done) {
     Level &L = mg->lv[l];
-    if (!L.n_full || L.own_ghosts) return;
-    const int gx = std::max(1, std::min(64, (L.max_full + 255) / 256));
-    hipLaunchKernelGGL(k_gather<T>, dim3(gx, L.n_full), dim3(256), 0, mg->stream, (T *)L.phi[L.cur], L.db,
-                       L.r_full, done);
+    if (L.own_ghosts) return;
+    fill_ghosts<T>(mg, L, (T *)L.phi[L.cur], done);
 }
 
 // The mixed form's FP32 inner cycle multiplies by 1/ap; the FP64 form
@@ -677,9 +796,14 @@ void restrict_to(mf3d_mg *mg, int lc, const int *done) {
     Level &C = mg->lv[lc], &F = mg->lv[lc - 1];
     const bool tk = tk_on(mg, MF3D_K_RESTRICT, lc - 1);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL((k_restrict<T, VAR>), dim3(C.n_in), dim3(kTileX, kTileY), 0, mg->stream,
-                       (const T *)F.phi[F.cur], (const T *)F.rhs, (const T *)F.cx, (const T *)F.cy,
-                       (const T *)F.cz, F.db, (T *)C.rhs, C.db, C.t_in, done);
+    if (C.n_rst)
+        hipLaunchKernelGGL((k_restrict<T, VAR>), dim3(C.n_rst), dim3(kTileX, kTileY), 0, mg->stream,
+                           (const T *)F.phi[F.cur], (const T *)F.rhs, (const T *)F.cx, (const T *)F.cy,
+                           (const T *)F.cz, F.db, (T *)C.rhs, C.db, C.t_rst, done);
+    if (C.n_rrep)  // blocks repeating their grid at lc (refined grids)
+        hipLaunchKernelGGL((k_restrict_rep<T, VAR>), dim3(C.n_rrep), dim3(kTileX, kTileY), 0, mg->stream,
+                           (const T *)F.phi[F.cur], (const T *)F.rhs, (const T *)F.cx, (const T *)F.cy,
+                           (const T *)F.cz, F.db, (T *)C.rhs, C.db, C.t_rrep, done);
     if (tk) tk_mark(mg);
 }
 
@@ -688,9 +812,14 @@ void prolong_to(mf3d_mg *mg, int lf, const int *done) {
     Level &C = mg->lv[lf + 1], &F = mg->lv[lf];
     const bool tk = tk_on(mg, MF3D_K_PROLONG, lf);
     if (tk) tk_mark(mg);
-    hipLaunchKernelGGL(k_prolong<T>, dim3(F.n_oct), dim3(kTileX, kTileY), 0, mg->stream,
-                       (const T *)F.phi[F.cur], (T *)F.phi[1 - F.cur], F.db, (const T *)C.phi[C.cur], C.db,
-                       F.t_oct, done);
+    if (F.n_oct)
+        hipLaunchKernelGGL(k_prolong<T>, dim3(F.n_oct), dim3(kTileX, kTileY), 0, mg->stream,
+                           (const T *)F.phi[F.cur], (T *)F.phi[1 - F.cur], F.db, (const T *)C.phi[C.cur], C.db,
+                           F.t_oct, done);
+    if (F.n_prep)  // blocks whose next level repeats their grid (refined grids)
+        hipLaunchKernelGGL(k_prolong_rep<T>, dim3(F.n_prep), dim3(kTileX, kTileY), 0, mg->stream,
+                           (const T *)F.phi[F.cur], (T *)F.phi[1 - F.cur], F.db, (const T *)C.phi[C.cur], C.db,
+                           F.t_prep, done);
     if (tk) tk_mark(mg);
     F.cur = 1 - F.cur;
 }
@@ -716,11 +845,7 @@ void coarse_smooth64(mf3d_mg *mg, int sweeps, const int *done) {
         if (s == 0) sweep_launch<double, VAR, true>(mg, S, done);
         else sweep_launch<double, VAR, false>(mg, S, done);
         S.cur = 1 - S.cur;
-        if (S.n_full && !S.own_ghosts) {
-            const int gx = std::max(1, std::min(64, (S.max_full + 255) / 256));
-            hipLaunchKernelGGL(k_gather<double>, dim3(gx, S.n_full), dim3(256), 0, mg->stream,
-                               (double *)S.phi[S.cur], S.db, S.r_full, done);
-        }
+        if (!S.own_ghosts) fill_ghosts<double>(mg, S, (double *)S.phi[S.cur], done);
     }
     convert<float, double>(mg, (float *)C.phi[1 - C.cur], (const double *)S.phi[S.cur], C.cells_gh);
     C.cur = 1 - C.cur;
@@ -738,10 +863,7 @@ void coarse_direct(mf3d_mg *mg, const int *done) {
                        (const double *)mg->d_cinv, mg->nc, (const CCell *)mg->d_ccell, C.db, (const T *)C.rhs, out,
                        done);
     C.cur = 1 - C.cur;
-    if (C.n_full) {
-        const int gx = std::max(1, std::min(64, (C.max_full + 255) / 256));
-        hipLaunchKernelGGL(k_gather<T>, dim3(gx, C.n_full), dim3(256), 0, mg->stream, out, C.db, C.r_full, done);
-    }
+    fill_ghosts<T>(mg, C, out, done);
 }
 
 // Levels lt..L-1 in one launch (k_tail); level lt's RHS is in place and its
@@ -821,6 +943,20 @@ void outer_step(mf3d_mg *mg, int from, const int *done, bool first = false) {
         hipLaunchKernelGGL((k_krylov_lds<0, double, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream,
                            mg->pbuf[from], mg->kd[1 - from], mg->pbuf[1 - from], mg->f_g, nullptr, mg->kr,
                            (T *)L0.rhs, cx, cy, cz, mg->st, L0.db, mg->t_out, mg->partials, done);
+        if (tk) tk_mark(mg);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts, mg->st,
+                           mg->d_hist, mg->d_rmax, mg->cfg.min_cycles);
+        return;
+    }
+    if (mg->refined) {  // p_new, its exchange, then r = f - A p_new
+        const unsigned gb = (unsigned)std::min<long long>(16384, (L0.cells_gh + 255) / 256);
+        hipLaunchKernelGGL(k_padd<T>, dim3(gb), dim3(256), 0, mg->stream, mg->pbuf[1 - from], mg->pbuf[from],
+                           (const T *)L0.phi[L0.cur], L0.cells_gh, done);
+        fill_ghosts<double>(mg, L0, mg->pbuf[1 - from], done);
+        if (tk) tk_mark(mg);
+        hipLaunchKernelGGL((k_outer_lds<T, VAR, 8, false>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream,
+                           mg->pbuf[1 - from], nullptr, (const T *)nullptr, mg->f_g, (T *)L0.rhs, cx, cy, cz, L0.db,
+                           mg->t_out, mg->partials, done);
         if (tk) tk_mark(mg);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts, mg->st,
                            mg->d_hist, mg->d_rmax, mg->cfg.min_cycles);
@@ -1308,6 +1444,10 @@ int build(mf3d_mg *mg) {
     const int L = c.levels, nb = c.nblocks;
     const bool mixed = c.precision == MF3D_MIXED;
     const size_t w = mixed ? 4 : 8;
+    mg->own = own_levels(nb, mg->blk.data(), L);
+    mg->refined = any_refined(nb, mg->blk.data());
+    // level l of block b repeats level l-1's grid (the block has fewer levels)
+    auto repeats = [&](int b, int l) { return l >= mg->own[b]; };
     mg->lv.resize(L);
     mg->cells = 0;
     for (int b = 0; b < nb; ++b) mg->cells += (long long)mg->blk[b].n[0] * mg->blk[b].n[1] * mg->blk[b].n[2];
@@ -1320,7 +1460,7 @@ int build(mf3d_mg *mg) {
         for (int b = 0; b < nb; ++b) {
             const mf3d_block_desc &B = mg->blk[b];
             BlockLevel &g = V.hb[b];
-            g.n0 = dim_at(B, 0, l); g.n1 = dim_at(B, 1, l); g.n2 = dim_at(B, 2, l);
+            g.n0 = dim_at(B, 0, l, mg->own[b]); g.n1 = dim_at(B, 1, l, mg->own[b]); g.n2 = dim_at(B, 2, l, mg->own[b]);
             n0max = std::max(n0max, g.n0);
             n1max = std::max(n1max, g.n1);
             g.neu = 0;
@@ -1332,8 +1472,9 @@ int build(mf3d_mg *mg) {
             g.poff = poff;
             off += g.sz * (g.n2 + 2);
             poff += (long long)g.n0 * g.n1 * g.n2;
-            // constant coefficients (mgsolver.for:51-57,184-189)
-            const double sc = (double)(1 << l);
+            // constant coefficients (mgsolver.for:51-57,184-189), the block's
+            // own grid level gl = min(glevel, ngrid)
+            const double sc = (double)(1 << std::min(l, mg->own[b] - 1));
             const double ndx = B.h[0] * sc, ndy = B.h[1] * sc, ndz = B.h[2] * sc;
             g.c0 = 1.0 / (ndx * ndx);
             g.c1 = 1.0 / (ndy * ndy);
@@ -1357,7 +1498,22 @@ int build(mf3d_mg *mg) {
         if ((rc = upload(mg, &V.db, V.hb))) return rc;
         auto tsw = xcd_order(sweep_tiles(V.hb, V.sw_ty)), tin = xcd_order(cell_tiles(V.hb)),
              trow = xcd_order(row_tiles(V.hb));
-        auto toct = xcd_order(octet_tiles(V.hb));
+        auto pick = [](const std::vector<Tile> &t, auto keep) {
+            std::vector<Tile> o;
+            for (const Tile &x : t)
+                if (keep(x.b)) o.push_back(x);
+            return o;
+        };
+        // restriction into l: coarsening blocks / blocks repeating their grid
+        auto trst = pick(tin, [&](int b) { return !repeats(b, l); });
+        auto trrep = pick(tin, [&](int b) { return repeats(b, l); });
+        // prolongation from l+1: blocks that repeat at l+1 take injection
+        auto toct = xcd_order(pick(octet_tiles(V.hb), [&](int b) { return !(l + 1 < L && repeats(b, l + 1)); }));
+        auto tprep = pick(trow, [&](int b) { return l + 1 < L && repeats(b, l + 1); });
+        V.n_rst = (int)trst.size(); V.n_rrep = (int)trrep.size(); V.n_prep = (int)tprep.size();
+        if ((rc = upload(mg, &V.t_rst, trst)) || (rc = upload(mg, &V.t_rrep, trrep)) ||
+            (rc = upload(mg, &V.t_prep, tprep)))
+            return rc;
         auto tch = xcd_order(chunk_tiles(V.hb));
         V.n_ch = (int)tch.size();
         if ((rc = upload(mg, &V.t_ch, tch))) return rc;
@@ -1366,7 +1522,19 @@ int build(mf3d_mg *mg) {
         if ((rc = upload(mg, &V.t_oct, toct))) return rc;
         if ((rc = upload(mg, &V.t_sw, tsw)) || (rc = upload(mg, &V.t_in, tin)) || (rc = upload(mg, &V.t_row, trow)))
             return rc;
-        auto rf = build_regions(mg->blk, l, &V.max_full);
+        std::vector<Region> rf;
+        if (mg->refined) {  // every ghost as an expression over interior cells
+            std::vector<GhostExpr> gx;
+            const std::string why = ghost_exprs(mg->blk, mg->own, L, l, &gx);
+            if (!why.empty()) {
+                mg->err = "level " + std::to_string(l) + ": " + why;
+                return MF3D_ENOTSUP;
+            }
+            V.n_gx = (int)gx.size();
+            if ((rc = upload(mg, &V.gx, gx))) return rc;
+        } else {
+            rf = build_regions(mg->blk, l, &V.max_full);
+        }
         V.n_full = (int)rf.size();
         // lone blocks (no neighbour, no periodic partner): each sweep stores the
         // ghost images itself and no gather runs.  MF3D_OWN_GHOSTS=0: gather.
@@ -1914,6 +2082,11 @@ int vel_grid(mf3d_mg *mg) {
 template <typename T, bool VAR>
 int newsolv_dev(mf3d_mg *mg, const VelFields &F, double dt, double alfapr, double eps, double *su,
                 double *rmax_out, int32_t *iters_out, double *hist) {
+    if (mg->refined) {
+        mg->err = "the velocity form on refined blocks (rdiv > 1) needs the 2:1 interpolations of "
+                  "exchangep/u/v/w, which this library does not have";
+        return MF3D_ENOTSUP;
+    }
     int rc = build_vel(mg);
     if (rc || (rc = vel_grid<T, VAR>(mg))) return rc;
     const int maxcy = mg->cfg.maxcy;
@@ -2096,6 +2269,11 @@ int mf3d_mg_create(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, mf3d_m
     if (!v.empty()) {
         mg->err = v;
         return MF3D_EINVAL;
+    }
+    v = refined_scope(*cfg, blocks);
+    if (!v.empty()) {
+        mg->err = v;
+        return MF3D_ENOTSUP;
     }
     mg->blk.assign(blocks, blocks + cfg->nblocks);
     int rc = set_device(mg);
@@ -2299,6 +2477,10 @@ int mf3d_mg_solve_device_noalloc(const mf3d_mg_cfg *cfg, const mf3d_block_desc *
         tmp.err = v;
         return fail(MF3D_EINVAL);
     }
+    if (any_refined(cfg->nblocks, blocks)) {
+        tmp.err = "mf3d_mg_solve_device_noalloc takes unrefined blocks only (rdiv 1)";
+        return fail(MF3D_ENOTSUP);
+    }
     if (cfg->variable_coef) {
         tmp.err = "mf3d_mg_solve_device_noalloc solves constant-coefficient problems only (a density "
                   "needs a handle: mf3d_mg_set_density)";
@@ -2397,6 +2579,33 @@ int mf3d_mg_newsolv(mf3d_mg *mg, const double *const *ustar, const double *const
 int64_t mf3d_mg_ghosted_cells(const mf3d_mg *mg) { return mg ? mg->lv[0].cells_gh : MF3D_EINVAL; }
 
 int64_t mf3d_mg_shell_cells(const mf3d_mg *mg) { return mg ? mg->cells_w : MF3D_EINVAL; }
+
+int mf3d_mg_ghost_table(const mf3d_mg_cfg *cfg, const mf3d_block_desc *blocks, int32_t level, int32_t *out,
+                        int64_t cap, int64_t *count, char *err, size_t err_len) {
+    auto fail = [&](int rc, const std::string &why) {
+        if (err && err_len) std::snprintf(err, err_len, "%s", why.c_str());
+        return rc;
+    };
+    if (err && err_len) err[0] = '\0';
+    if (!cfg || !blocks || !count || (cap > 0 && !out)) return fail(MF3D_EINVAL, "null argument");
+    std::string v = validate(*cfg, blocks);
+    if (!v.empty()) return fail(MF3D_EINVAL, v);
+    v = refined_scope(*cfg, blocks);
+    if (!v.empty()) return fail(MF3D_ENOTSUP, v);
+    if (level < 0 || level >= cfg->levels) return fail(MF3D_EINVAL, "level out of range");
+    std::vector<mf3d_block_desc> blk(blocks, blocks + cfg->nblocks);
+    std::vector<GhostExpr> gx;
+    v = ghost_exprs(blk, own_levels(cfg->nblocks, blocks, cfg->levels), cfg->levels, level, &gx);
+    if (!v.empty()) return fail(MF3D_ENOTSUP, v);
+    *count = (int64_t)gx.size();
+    for (int64_t i = 0; i < std::min<int64_t>(cap, *count); ++i) {
+        const GhostExpr &e = gx[(size_t)i];
+        out[10 * i] = e.dst;
+        out[10 * i + 1] = e.kind;
+        for (int q = 0; q < 8; ++q) out[10 * i + 2 + q] = e.src[q];
+    }
+    return MF3D_OK;
+}
 
 int mf3d_mg_time_kernel(mf3d_mg *mg, int kernel) {
     if (!mg || kernel < MF3D_K_OFF || kernel > MF3D_K_HEAVISIDE) return MF3D_EINVAL;

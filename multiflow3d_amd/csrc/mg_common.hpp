@@ -53,6 +53,15 @@ struct Region {
     int map[3];
 };
 
+// One ghost of a refined block grid's level: kind 0 copies src[0]; 1 takes
+// 0.125 x (src[0] + ... + src[7]) summed in that order; 2 takes
+// ((2 src[0] + 3 src[1]) - src[2]) / 4.  Offsets index the level arena;
+// every src is an interior cell.
+struct GhostExpr {
+    int dst, kind;
+    int src[8];
+};
+
 struct SolveState {
     double fnorm;
     int done;   // converged: later cycle kernels return at once

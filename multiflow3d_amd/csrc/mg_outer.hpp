@@ -17,7 +17,11 @@ namespace mf3d {
 // in x and y).  Each workgroup writes p_new for its own cells plus the ghost
 // cells next to them on the block boundary.  Plane c+2 of p_old and e and
 // the next plane's f are loaded while plane c is computed.
-template <typename TE, bool VAR, int TY>
+//
+// ADD = false (refined block grids, whose 2:1 ghosts are not the sum of the
+// ghosts of p_old and e): pold already holds p_new with its ghosts; only r
+// and the partials are formed (e and pnew unused).
+template <typename TE, bool VAR, int TY, bool ADD = true>
 __global__ __launch_bounds__(SW_TX *TY) void k_outer_lds(
     const double *__restrict__ pold, double *__restrict__ pnew, const TE *__restrict__ e,
     const double *__restrict__ f, TE *__restrict__ rhs0, const double *__restrict__ cx,
@@ -66,9 +70,11 @@ __global__ __launch_bounds__(SW_TX *TY) void k_outer_lds(
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             const double a = pold[lsrc[k] + zo];
-            const TE b = e[lsrc[k] + zo];
             R.pp[k] = lok[k] ? a : 0.0;
-            R.pe[k] = lok[k] ? b : TE(0);
+            if constexpr (ADD) {
+                const TE b = e[lsrc[k] + zo];
+                R.pe[k] = lok[k] ? b : TE(0);
+            }
         }
         const bool ok = has_col && c >= t.z0 && c <= t.z1;
         const double v = f[ok ? gidx(g, bx, by, c) : g.off];
@@ -80,9 +86,13 @@ __global__ __launch_bounds__(SW_TX *TY) void k_outer_lds(
 #pragma unroll
         for (int k = 0; k < KL; ++k) {
             if (tid + k * NT >= LXY) continue;
-            const double v = pp[k] + (double)pe[k];
-            if (zown && lown[k]) pnew[lofs[k] + g.sz * c] = v;
-            P[tid + k * NT] = v;
+            if constexpr (ADD) {
+                const double v = pp[k] + (double)pe[k];
+                if (zown && lown[k]) pnew[lofs[k] + g.sz * c] = v;
+                P[tid + k * NT] = v;
+            } else {
+                P[tid + k * NT] = pp[k];
+            }
         }
     };
     double r2 = 0.0, ra = 0.0;
@@ -132,6 +142,16 @@ __global__ __launch_bounds__(SW_TX *TY) void k_outer_lds(
     if (c <= t.z1) step(c, R0);
     if (c + 1 <= t.z1) step(c + 1, R1);
     block_sum_max<SW_TX * TY>(r2, ra, partials + 2 * (long long)blockIdx.x);
+}
+
+// p_new = p_old + e on every cell of the level-0 arena (a refined grid's
+// outer step; its ghosts are then refilled from p_new by the exchange).
+template <typename TE>
+__global__ __launch_bounds__(256) void k_padd(double *__restrict__ pnew, const double *__restrict__ pold,
+                                              const TE *__restrict__ e, long long n, const int *__restrict__ done) {
+    if (done && *done) return;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        pnew[i] = pold[i] + (double)e[i];
 }
 
 // ---------------------------------------------------------------------------

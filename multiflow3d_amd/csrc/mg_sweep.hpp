@@ -588,4 +588,28 @@ __global__ __launch_bounds__(256) void k_gather(T *__restrict__ phi,
     }
 }
 
+// Ghost fill of a refined block grid's level from its expression table
+// (every ghost is a copy, an 8-cell average or a 2:1 extrapolation of
+// interior cells, which this kernel never writes).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_expr(T *__restrict__ phi, const GhostExpr *__restrict__ ex, int n,
+                                                     const int *__restrict__ done) {
+    if (done && *done) return;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const GhostExpr e = ex[t];
+        T v;
+        if (e.kind == 0) {
+            v = phi[e.src[0]];
+        } else if (e.kind == 1) {
+            T s = phi[e.src[0]];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) s = s + phi[e.src[q]];
+            v = T(0.125) * s;
+        } else {
+            v = ((T(2) * phi[e.src[0]] + T(3) * phi[e.src[1]]) - phi[e.src[2]]) / T(4);
+        }
+        phi[e.dst] = v;
+    }
+}
+
 }  // namespace mf3d

@@ -49,6 +49,24 @@ __global__ __launch_bounds__(256) void k_restrict(
     rhsc[gidx(gc, i, j, k)] = T(0.125) * s;
 }
 
+// Restriction into a level where a block repeats its grid (a refined grid's
+// block past its own level count): the coarse RHS is the fine residual,
+// cell for cell (mgsolver.for:719-732).  Tiles: 64 x 4 interior cells of
+// one plane of such blocks.
+template <typename T, bool VAR>
+__global__ __launch_bounds__(256) void k_restrict_rep(
+    const T *__restrict__ phif, const T *__restrict__ rhsf, const T *__restrict__ cx,
+    const T *__restrict__ cy, const T *__restrict__ cz, const BlockLevel *__restrict__ blf,
+    T *__restrict__ rhsc, const BlockLevel *__restrict__ blc, const Tile *__restrict__ tiles,
+    const int *__restrict__ done) {
+    if (done && *done) return;
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel gc = blc[t.b];
+    const int i = t.x0 + threadIdx.x, j = t.y0 + threadIdx.y, k = t.z0;
+    if (i > gc.n0 || j > gc.n1) return;
+    rhsc[gidx(gc, i, j, k)] = residual_at<T, VAR>(phif, rhsf, cx, cy, cz, blf[t.b], i, j, k);
+}
+
 // Trilinear prolongation added to every fine point including the ghost
 // layer (mgsolver.for:823-891), fused with the Neumann copy that follows it
 // (mgbound, mgsolver.for:902): a ghost on a physical Neumann face takes the
@@ -119,6 +137,33 @@ __global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ fin, T *_
                     s = s + (wi1 * wj1[dy] * wk1[dz]) * c111;
                     fout[gidx(gf, i, 2 * my + dy, 2 * mz + dz)] = fin[gidx(gf, si, sj[dy], sk[dz])] + s / T(64);
                 }
+    }
+}
+
+// Correction of a block whose next level repeats its grid (mgsolver.for:
+// 807-819): every cell, ghosts included, adds the coarse value of the same
+// cell, fused with the Neumann copy that follows as in k_prolong (a ghost on
+// a physical Neumann face takes the corrected value of the cell it
+// reflects).  Reads fin, writes fout.  Tiles: 4 full ghosted rows of one
+// ghosted plane of such blocks.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_rep(const T *__restrict__ fin, T *__restrict__ fout,
+                                                     const BlockLevel *__restrict__ blf,
+                                                     const T *__restrict__ phic,
+                                                     const BlockLevel *__restrict__ blc,
+                                                     const Tile *__restrict__ tiles,
+                                                     const int *__restrict__ done) {
+    if (done && *done) return;
+    const Tile t = tiles[blockIdx.x];
+    const BlockLevel gf = blf[t.b], gc = blc[t.b];
+    const int j = t.y0 + threadIdx.y, k = t.z0;
+    if (j > gf.n1 + 1) return;
+    auto refl = [](int v, int n, bool lo, bool hi) { return (v == 0 && lo) ? 1 : ((v == n + 1 && hi) ? n : v); };
+    const int sj = refl(j, gf.n1, gf.neu & kNeuS, gf.neu & kNeuN);
+    const int sk = refl(k, gf.n2, gf.neu & kNeuB, gf.neu & kNeuT);
+    for (int i = threadIdx.x; i <= gf.n0 + 1; i += blockDim.x) {
+        const int si = refl(i, gf.n0, gf.neu & kNeuW, gf.neu & kNeuE);
+        fout[gidx(gf, i, j, k)] = fin[gidx(gf, si, sj, sk)] + phic[gidx(gc, si, sj, sk)];
     }
 }
 

@@ -24,6 +24,7 @@ module mf3d_mg_f
     integer(c_int32_t) :: nbr(6)     ! iprev, inext, jprev, jnext, kprev, knext
     integer(c_int32_t) :: per(6)     ! per_ip, per_in, per_jp, per_jn, per_kp, per_kn
     integer(c_int32_t) :: bc(6)      ! bc_west .. bc_top
+    integer(c_int32_t) :: rdiv = 1   ! rdiv(dom_id(ib)); n and h at the block's own resolution
   end type
 
   ! the MG scalars of control.cin (initial.for:20-26)

@@ -67,6 +67,7 @@ def _descs(blocks: Sequence[Block], ghost: int):
         arr[i].nbr[:] = list(b.nbr)
         arr[i].per[:] = list(b.per)
         arr[i].bc[:] = list(b.bc)
+        arr[i].rdiv = int(getattr(b, "rdiv", 1))
     return arr
 
 

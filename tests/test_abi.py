@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
                         text=True, check=True).stdout
     exported = set(re.findall(r"\sT\s(mf3d_\w+)", nm))
     assert declared <= exported
-    assert L.mf3d_mg_abi_version() == 6
+    assert L.mf3d_mg_abi_version() == 7
 
 
 def test_product_library_does_not_contain_the_oracle():

@@ -1,0 +1,151 @@
+"""GPU parity on refined block grids (rdiv > 1), through the C ABI.
+
+- Every refined fixture of the compiled reference (rows of blocks along x,
+  y or z, rdiv 1/2 and 2/4 side by side, walls or a block's own periodic
+  wrap on the other axes): FP64 pp after one V-cycle with its ghost layer
+  and p after the fixed cycles bitwise the reference's, the residual
+  history to 1e-12 (the GPU sums the norm in a different order).
+- The same cycles with the 3-D tile sweep forced on every level, and with
+  the march forced: bitwise too.
+- Solves to a tolerance need a right-hand side in the operator's range:
+  with walls and 2:1 faces the operator is singular and not symmetric, so
+  a mean-free RHS is not enough (the fixtures' RHS leaves 3.5 % of its norm
+  outside the range, where the residual stops).  f = A p_true (the oracle's
+  operator, its 2:1 ghosts included) is.  Mixed precision (FP32 inner
+  cycle, FP64 outer step and coarsest level) and FP64 on the fixtures'
+  grids to 1e-9: within 1e-7 of each other and 1e-6 of p_true (mean
+  removed; at a 1e-9 residual the oracle itself is 1e-9 .. 1.5e-7 from it).
+- A larger refined row (a 64 x 32 x 32 coarse block beside a 64^3 refined
+  one, 5 levels): bitwise against the oracle for 2 cycles in FP64, then
+  both precisions to 1e-8 on f = A p_true.
+"""
+import json
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+from multiflow3d_amd import NotConverged, PressureMultigrid
+from multiflow3d_amd import topology as T
+from multiflow3d_amd import workloads as W
+from oracle.oracle import OracleMG
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MAN = json.load(open(os.path.join(GOLDEN, "manifest_refined.json")))
+
+
+def _load(name):
+    m = MAN[name]
+    g = T.refined_grid(m["ex"], m["ey"], m["ez"], m["bc"], m["h"], m["rdiv"])
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+
+    def split(flat, ghost=0):
+        out, o = [], 0
+        for b in g.blocks:
+            nx, ny, nz = (x + 2 * ghost for x in b.n)
+            out.append(flat[o:o + nx * ny * nz].reshape(nz, ny, nx))
+            o += nx * ny * nz
+        return out
+
+    return g, m, dict(f=split(z["f"]), p=split(z["p"]), pp1=split(z["pp1"], 1), rho=z["rho"])
+
+
+def _solve(g, L, f, **kw):
+    with PressureMultigrid(g, L, **kw) as mg:
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", NotConverged)
+            return mg.newsolv_mg(f)
+
+
+@pytest.mark.parametrize("name", sorted(MAN))
+def test_refined_fp64_matches_reference_fixture(name):
+    g, m, a = _load(name)
+    print(f"[refined fp64] {name}", flush=True)
+    with PressureMultigrid(g, m["levels"]) as mg:
+        pp = [np.zeros((b.n[2] + 2, b.n[1] + 2, b.n[0] + 2)) for b in g.blocks]
+        mg.mgkcyc(a["f"], pp)
+    for x, e in zip(pp, a["pp1"]):
+        assert np.array_equal(x, e), np.abs(x - e).max()
+    p, res = _solve(g, m["levels"], a["f"], maxcy=m["cycles"], eps=0.0)
+    assert res.iters == m["cycles"]
+    np.testing.assert_allclose(res.hist, a["rho"], rtol=1e-12, atol=0)
+    for x, e in zip(p, a["p"]):
+        assert np.array_equal(x, e), np.abs(x - e).max()
+
+
+@pytest.mark.parametrize("tiles", ["3d", "march"])
+@pytest.mark.parametrize("name", ["ref_x_cfc_yper_L3", "ref_x_r2r4_L4"])
+def test_refined_both_sweep_kernels_bitwise(name, tiles, monkeypatch):
+    g, m, a = _load(name)
+    print(f"[refined sweep {tiles}] {name}", flush=True)
+    monkeypatch.setenv("MF3D_SWEEP_3D", "1" if tiles == "3d" else "0")
+    p, res = _solve(g, m["levels"], a["f"], maxcy=m["cycles"], eps=0.0)
+    for x, e in zip(p, a["p"]):
+        assert np.array_equal(x, e)
+
+
+def _rel(a, b):
+    a = np.concatenate([x.ravel() for x in a])
+    b = np.concatenate([x.ravel() for x in b])
+    a, b = a - a.mean(), b - b.mean()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _in_range(g, L, seed):
+    """(f, p_true) with f = A p_true on level 0, p_true smooth plus noise."""
+    rng = np.random.default_rng(seed)
+    o = OracleMG(g.blocks, L)
+    o.coeff()
+    pt = []
+    for b, blk in enumerate(g.blocks):
+        ph = o.phi(b, 0)
+        z, y, x = np.meshgrid(*[(np.arange(n) + 0.5) / n for n in blk.n[::-1]], indexing="ij")
+        v = np.cos(np.pi * x) * np.cos(np.pi * y) + 0.1 * rng.standard_normal(x.shape)
+        ph[1:-1, 1:-1, 1:-1] = v
+        o.rhs(b, 0)[...] = 0.0
+        pt.append(v)
+    o.exchange(0)
+    f = [-r for r in o.residual(0)]
+    o.close()
+    return f, pt
+
+
+@pytest.mark.parametrize("name", ["ref_x_cf_walls_L4", "ref_y_cf_xper_L3", "ref_x_r2r4_L4"])
+def test_refined_mixed_within_tolerance_of_fp64(name):
+    g, m, _ = _load(name)
+    print(f"[refined mixed] {name}", flush=True)
+    f, pt = _in_range(g, m["levels"], 9)
+    p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9)
+    pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, precision="mixed")
+    assert r64.status == 0 and rmx.status == 0 and rmx.hist[-1] <= 1e-9
+    err, e64 = _rel(pmx, p64), _rel(p64, pt)
+    print(f"  fp64 {r64.iters} / mixed {rmx.iters} cycles, mixed vs fp64 {err:.2e}, fp64 vs p_true {e64:.2e}",
+          flush=True)
+    assert err < 1e-7 and e64 < 1e-6
+
+
+@pytest.mark.timeout(240)
+def test_refined_larger_row():
+    g = T.refined_grid([64, 32], [32], [32], [T.NEUMANN] * 4 + [T.PERIODIC] * 2, [1 / 96] * 3, [1, 2])
+    L = 5
+    f = W.block_rhs(g, W.BASE_SEED + 70)
+    print("[refined larger row]", [b.n for b in g.blocks], flush=True)
+    o = OracleMG(g.blocks, L)
+    o.coeff()
+    p_o, rho_o, _, _ = o.solve(f, 0.0, 2)
+    o.close()
+    p2, r2 = _solve(g, L, f, maxcy=2, eps=0.0)
+    np.testing.assert_allclose(r2.hist, rho_o, rtol=1e-12, atol=0)
+    for x, e in zip(p2, p_o):
+        assert np.array_equal(x, e)
+    f, pt = _in_range(g, L, 10)
+    p64, r64 = _solve(g, L, f, maxcy=120, eps=1e-8)
+    pmx, rmx = _solve(g, L, f, maxcy=120, eps=1e-8, precision="mixed")
+    assert r64.status == 0 and rmx.status == 0
+    err, e64 = _rel(pmx, p64), _rel(p64, pt)
+    print(f"  fp64 {r64.iters} / mixed {rmx.iters} cycles, mixed vs fp64 {err:.2e}, fp64 vs p_true {e64:.2e}",
+          flush=True)
+    assert err < 1e-6 and e64 < 1e-4
