@@ -262,7 +262,12 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
         nx, ny, nz = blk.n
         m = (nx + 2) * (ny + 2) * (nz + 2)
         pp1.append(raw[o:o + m].reshape(nz + 2, ny + 2, nx + 2).copy()); o += m
+    global LAST_REF_SECONDS
+    LAST_REF_SECONDS = float(raw[o]) if o < len(raw) else None
     return p, rho[:k + 1].copy(), rmax[:k + 1].copy(), k, pp1
+
+
+LAST_REF_SECONDS = None  # solve time of the last run_ref, measured inside the driver
 
 
 def have_ref_vel() -> bool:

@@ -22,7 +22,7 @@
 !            extents are its column's extents x rdiv, its spacing dx / rdiv
 ! Output (f64 stream): ncycles, rho(0:maxcy), rmax(0:maxcy), then per block
 ! the interior of p, then per block pp after cycle 1 including one ghost
-! layer.
+! layer, then the seconds the solve took (the loop, without input reading).
 program mg_ref_driver
   use vars
   use multidata
@@ -39,6 +39,7 @@ program mg_ref_driver
   character(len=1024) :: ctl, frhs, fdens, fout
   type(fieldp), allocatable :: f(:), pp1(:)
   integer :: mi, mj, mk, nwork, pre, tot, u, ios, maxr, gl, r
+  integer(8) :: clk0, clk1, clkrate
   integer, allocatable :: rdivs(:)
 
   call get_command_argument(1, ctl)
@@ -200,6 +201,7 @@ program mg_ref_driver
 
   allocate(rho(0:maxcy), rmx(0:maxcy))
   rho = 0.0; rmx = 0.0
+  call system_clock(clk0, clkrate)
   call residual(nf, rm, .true.)
   call residual(nr, rm, .false.)
   if (nf == 0.0) nf = 1.0
@@ -234,6 +236,7 @@ program mg_ref_driver
     end if
   end do
 
+  call system_clock(clk1)
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) dble(ncyc)
   write(u) rho
@@ -244,6 +247,7 @@ program mg_ref_driver
   do ib = 1, nbp
     write(u) pp1(ib)%a
   end do
+  write(u) dble(clk1 - clk0) / dble(clkrate)   ! seconds of the solve (after the input is read)
   close(u)
 
 contains
