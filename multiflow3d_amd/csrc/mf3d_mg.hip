@@ -71,6 +71,11 @@ struct Level {
     int adi_cells = 0, adi_sstride = 0;
     void *adi_apr[3] = {}, *adi_a[3] = {};  // the LDS kernel's Thomas coefficients per line axis
     int sw_ty = 4;  // sweep tile rows of the march; 0: 3-D tiles (k_rb3d)
+    // the zero-start march (k_rbz) on its own chunks: the default depth
+    // (the deepest that still gives ~1024 workgroups), which suits its
+    // light steps better than the full sweep's tuned depth
+    Tile *t_swz = nullptr;
+    int n_swz = 0, swz_ty = 4;
     Region *r_full = nullptr;
     int n_full = 0, max_full = 0;
     // refined block grids: the level's ghosts as expressions (no regions)
@@ -690,14 +695,14 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
         return;
     }
     if (ZERO && mg->zmode == 1) {  // from zero: the march over the RHS alone
-        if (L.sw_ty == 8)
-            hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pout,
+        if (L.swz_ty == 8)
+            hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 8>), dim3(L.n_swz), dim3(SW_TX, 8), 0, mg->stream, pout,
                                (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
-                               L.t_sw, done);
+                               L.t_swz, done);
         else
-            hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pout,
+            hipLaunchKernelGGL((k_rbz<T, VAR, RCP, 4>), dim3(L.n_swz), dim3(SW_TX, 4), 0, mg->stream, pout,
                                (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
-                               L.t_sw, done);
+                               L.t_swz, done);
         return;
     }
     if (L.sw_ty == 8)
@@ -1520,6 +1525,9 @@ int build(mf3d_mg *mg) {
         V.n_sw = (int)tsw.size(); V.n_in = (int)tin.size(); V.n_row = (int)trow.size();
         V.n_oct = (int)toct.size();
         if ((rc = upload(mg, &V.t_oct, toct))) return rc;
+        V.swz_ty = V.sw_ty;
+        V.n_swz = (int)tsw.size();
+        if ((rc = upload(mg, &V.t_swz, tsw))) return rc;
         if ((rc = upload(mg, &V.t_sw, tsw)) || (rc = upload(mg, &V.t_in, tin)) || (rc = upload(mg, &V.t_row, trow)))
             return rc;
         std::vector<Region> rf;

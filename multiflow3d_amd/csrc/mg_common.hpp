@@ -153,47 +153,63 @@ template <> __device__ __forceinline__ void bstore<float>(__amdgpu_buffer_rsrc_t
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, v), r, voff, soff, 0);
 }
 
-// Fixed-shape tree reduction of (sum, sum) over a workgroup of NT threads.
-template <int NT>
-__device__ __forceinline__ void block_sum2(double a, double b, double *out) {
-    __shared__ double sa[NT], sb[NT];
+// Wavefront reductions over the 64 lanes by a butterfly: each step combines
+// lanes l and l^o, so every lane ends with the same bits (the pairwise
+// operations commute) and the order never changes -- results are bitwise
+// reproducible run to run.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Workgroup reduction of two values over NT threads (a multiple of 64, at
+// most 4096): a wavefront reduction, one LDS slot per wave, one barrier,
+// and a wavefront reduction of those slots.  a is summed; b is summed, or
+// (MAX) maximised -- of values >= 0.  The results are valid in the first
+// wavefront (threads 0..63).  Every thread of the workgroup must call it.
+template <int NT, bool MAX>
+__device__ __forceinline__ void wg_reduce2(double &a, double &b) {
+    constexpr int NW = NT / 64;
+    static_assert(NW >= 1 && NW <= 64 && NW * 64 == NT, "workgroup of whole wavefronts");
+    __shared__ double wa[NW], wb[NW];
     const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    sa[t] = a;
-    sb[t] = b;
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            sa[t] = sa[t] + sa[t + w];
-            sb[t] = sb[t] + sb[t + w];
-        }
-        __syncthreads();
+    a = wave_sum(a);
+    b = MAX ? wave_max(b) : wave_sum(b);
+    if ((t & 63) == 0) {
+        wa[t >> 6] = a;
+        wb[t >> 6] = b;
     }
-    if (t == 0) {
-        out[0] = sa[0];
-        out[1] = sb[0];
+    __syncthreads();
+    if (t < 64) {
+        a = t < NW ? wa[t] : 0.0;
+        b = t < NW ? wb[t] : 0.0;
+        a = wave_sum(a);
+        b = MAX ? wave_max(b) : wave_sum(b);
     }
 }
 
-// Fixed-shape tree reduction of (sum, max) over a workgroup of NT threads
-// (a power of two); the order never changes, so results are bitwise
-// reproducible run to run.
+// Per-workgroup partials: (sum, sum) and (sum, max) of NT threads into out.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double a, double b, double *out) {
+    wg_reduce2<NT, false>(a, b);
+    if (threadIdx.x + threadIdx.y * blockDim.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
 template <int NT>
 __device__ __forceinline__ void block_sum_max(double s, double m, double *out) {
-    __shared__ double ss[NT], sm[NT];
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
-    ss[t] = s;
-    sm[t] = m;
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            ss[t] = ss[t] + ss[t + w];
-            sm[t] = fmax(sm[t], sm[t + w]);
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        out[0] = ss[0];
-        out[1] = sm[0];
+    wg_reduce2<NT, true>(s, m);
+    if (threadIdx.x + threadIdx.y * blockDim.x == 0) {
+        out[0] = s;
+        out[1] = m;
     }
 }
 

@@ -280,29 +280,19 @@ template <bool BETA>
 __global__ __launch_bounds__(1024) void k_gcr_scalar(const double *__restrict__ partials, int nparts,
                                                      SolveState *st) {
     if (st->done) return;
-    __shared__ double sa[1024], sb[1024];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0;
     for (int p = t; p < nparts; p += 1024) {
         a = a + partials[2 * (long long)p];
         b = b + partials[2 * (long long)p + 1];
     }
-    sa[t] = a;
-    sb[t] = b;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) {
-            sa[t] = sa[t] + sa[t + w];
-            sb[t] = sb[t] + sb[t + w];
-        }
-        __syncthreads();
-    }
+    wg_reduce2<1024, false>(a, b);
     if (t == 0) {
         if (BETA) {
-            st->beta = (st->fk > 0 && st->gamma != 0.0) ? sa[0] / st->gamma : 0.0;
+            st->beta = (st->fk > 0 && st->gamma != 0.0) ? a / st->gamma : 0.0;
         } else {
-            st->gamma = sa[0];
-            st->alpha = sa[0] != 0.0 ? sb[0] / sa[0] : 0.0;
+            st->gamma = a;
+            st->alpha = a != 0.0 ? b / a : 0.0;
             st->fk = st->fk + 1;
         }
     }
@@ -315,30 +305,20 @@ __global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ pa
                                                    SolveState *st, double *hist, double *rmaxh,
                                                    int min_cycles) {
     if (st->done) return;
-    __shared__ double ss[1024], sm[1024];
     const int t = threadIdx.x;
     double s = 0.0, m = 0.0;
     for (int p = t; p < nparts; p += 1024) {
         s = s + partials[2 * (long long)p];
         m = fmax(m, partials[2 * (long long)p + 1]);
     }
-    ss[t] = s;
-    sm[t] = m;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) {
-            ss[t] = ss[t] + ss[t + w];
-            sm[t] = fmax(sm[t], sm[t + w]);
-        }
-        __syncthreads();
-    }
+    wg_reduce2<1024, true>(s, m);
     if (t == 0) {
         const int k = st->k;
-        const double nr = sqrt(ss[0]);
+        const double nr = sqrt(s);
         if (k == 0) st->fnorm = nr > 0.0 ? nr : 1.0;
         const double rho = nr / st->fnorm;
         hist[k] = rho;
-        rmaxh[k] = sm[0];
+        rmaxh[k] = m;
         st->last = k;
         st->k = k + 1;
         if (k >= 1 && rho <= st->tol && k >= min_cycles) st->done = 1;

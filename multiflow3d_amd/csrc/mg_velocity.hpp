@@ -315,16 +315,9 @@ __global__ __launch_bounds__(SW_TX *TY, LSM ? 4 : 6) void k_velmas(
         }
     }
     if (!partials) return;
-    static_assert(VEL_TZ_MAX * SW_TX >= NT, "the reduction reuses the south halo");
-    double *sm = &sS[0][0];
-    __syncthreads();  // the last tile's halo reads are done
-    sm[tid] = rm;
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-        if (tid < w) sm[tid] = fmax(sm[tid], sm[tid + w]);
-        __syncthreads();
-    }
-    if (tid == 0) partials[blockIdx.x] = sm[0];
+    double unused = 0.0;
+    wg_reduce2<NT, true>(unused, rm);
+    if (tid == 0) partials[blockIdx.x] = rm;
 }
 
 // calmas, 4th-order branch (differencing = 2, press.for:13-36), after
@@ -390,22 +383,16 @@ __global__ void k_vel_reset(SolveState *st, double eps, double dta, double dt) {
 __global__ __launch_bounds__(1024) void k_vel_finalize(const double *__restrict__ partials, int nparts,
                                                        SolveState *st, double *rmaxh) {
     if (st->done) return;
-    __shared__ double sm[1024];
     const int t = threadIdx.x;
-    double m = 0.0;
+    double m = 0.0, unused = 0.0;
     for (int q = t; q < nparts; q += 1024) m = fmax(m, partials[q]);
-    sm[t] = m;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (t < w) sm[t] = fmax(sm[t], sm[t + w]);
-        __syncthreads();
-    }
+    wg_reduce2<1024, true>(unused, m);
     if (t == 0) {
         const int k = st->k;
-        rmaxh[k] = sm[0];
+        rmaxh[k] = m;
         st->last = k;
         st->k = k + 1;
-        if (sm[0] < st->tol && k > 1) st->done = 1;
+        if (m < st->tol && k > 1) st->done = 1;
     }
 }
 
