@@ -172,3 +172,35 @@ def test_invalid_pinned_depth_is_rejected(monkeypatch):
     grid, L, lsm = case("multi_uneven_L3")
     with pytest.raises(Exception, match="MF3D_SWEEP_TZ"):
         PressureMultigrid(grid, L)
+
+
+@pytest.mark.parametrize("nu", [(2, 2), (1, 1), (2, 1), (1, 0), (3, 2)])
+@pytest.mark.parametrize("name", ["single32_L5", "multi_uneven_L3", "periodic_multi_L3", "mixed_bc_multi_L3"])
+def test_cycle_shapes_bitwise(name, nu):
+    """V(nu_pre, nu_post) cycles other than the default V(2,2), including
+    ones that toggle the level-0 potential an even number of times (nu_pre
+    + 1 + nu_post): FP64 bitwise the oracle's, through the cycle graphs
+    and through the timing hook's launches (timed graphs on the first
+    solve, eager launches where graphs cannot record events)."""
+    from multiflow3d_amd import _lib
+
+    grid, L, _ = case(name)
+    f, _ = inputs(grid, False, seed=17)
+    ncyc = 5
+    o = OracleMG(grid.blocks, L, nu_pre=nu[0], nu_post=nu[1])
+    o.coeff()
+    p_o, rho_o, _, _ = o.solve(f, 0.0, ncyc)
+    for hook in (False, True):
+        with _gpu(grid, L, False, None, maxcy=ncyc, eps=0.0, nu_pre=nu[0], nu_post=nu[1]) as mg:
+            if hook:
+                mg.time_kernel(_lib.MF3D_K_OUTER)
+            for run in range(2 if hook else 1):
+                with warnings.catch_warnings():
+                    warnings.simplefilter("ignore", NotConverged)
+                    p_g, res = mg.newsolv_mg(f)
+                assert res.iters == ncyc
+                np.testing.assert_allclose(res.hist, rho_o, rtol=1e-12, atol=0)
+                for a, b in zip(p_g, p_o):
+                    assert np.array_equal(a, b), f"{name}/{nu}/hook={hook}/{run}: max {np.max(np.abs(a - b))}"
+            if hook:
+                assert mg.kernel_time()[0] >= 1
