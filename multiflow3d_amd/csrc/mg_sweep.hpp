@@ -360,9 +360,31 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbz(T *__restrict__ pout, const T
     const uint32_t o_ring = ob + (uint32_t)(in_r ? xr + sy * yr : sy + 1) * W;
     const int bix = own_ghosts && in ? ghost_image(x, g.n0, g.neu & kNeuW, g.neu & kNeuE) : -1;
     const int biy = own_ghosts && in ? ghost_image(y, g.n1, g.neu & kNeuS, g.neu & kNeuN) : -1;
+    // Constant coefficients with reciprocals (the mixed form): 1/ap of the
+    // own and the ring cell in the three plane classes (first, inner, last
+    // plane), formed once; 1/ap is the same correctly rounded quotient
+    // wherever it is formed, so the results are unchanged.
+    constexpr bool HOIST = RCP && !VAR;
+    T iao[3] = {T(0), T(0), T(0)}, iar[3] = {T(0), T(0), T(0)};
+    if constexpr (HOIST) {
+        const int kc[3] = {1, g.n2 > 2 ? 2 : 1, g.n2};
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            T aw, ae, as, an, ab, at, ap;
+            coefs<T, false>(g, cx, cy, cz, 0, in ? x : 1, in ? y : 1, kc[m], aw, ae, as, an, ab, at, ap);
+            iao[m] = T(1) / ap;
+            coefs<T, false>(g, cx, cy, cz, 0, in_r ? xr : 1, in_r ? yr : 1, kc[m], aw, ae, as, an, ab, at, ap);
+            iar[m] = T(1) / ap;
+        }
+    }
+    auto pcls = [&](int cc) { return cc == 1 ? 0 : (cc == g.n2 ? 2 : 1); };
     // first-colour value of (xx, yy, cc) from its RHS r, 0 elsewhere
-    auto redval = [&](bool ok, int xx, int yy, int cc, T r) -> T {
+    auto redval = [&](bool ok, int xx, int yy, int cc, T r, const T *ia) -> T {
         if (!ok || cc < 1 || cc > g.n2 || !((xx + yy + cc) & 1)) return T(0);
+        if constexpr (HOIST) {
+            const int m = pcls(cc);
+            return (r - T(0)) * (m == 0 ? ia[0] : (m == 2 ? ia[2] : ia[1]));
+        }
         T aw, ae, as, an, ab, at, ap;
         coefs<T, VAR>(g, cx, cy, cz, gidx(g, xx, yy, cc), xx, yy, cc, aw, ae, as, an, ab, at, ap);
         return RCP ? (r - T(0)) * (T(1) / ap) : (r - T(0)) / ap;
@@ -388,8 +410,8 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbz(T *__restrict__ pout, const T
             const T rcur = R[J], rring = RR[J];
             R[J] = ld(o_own, c + NS);
             RR[J] = ld(o_ring, c + NS);
-            red[J][lo] = redval(in, x, y, c, rcur);
-            if (has_ring) red[J][lr] = redval(in_r, xr, yr, c, rring);
+            red[J][lo] = redval(in, x, y, c, rcur, iao);
+            if (has_ring) red[J][lr] = redval(in_r, xr, yr, c, rring, iar);
             __syncthreads();
             const int z = c - 1;
             if (in && z >= t.z0 && z <= t.z1) {
@@ -406,7 +428,12 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbz(T *__restrict__ pout, const T
                     s = madd(s, an, P[lo + LX]);
                     s = madd(s, ab, red[(J + NS - 2) % NS][lo]);
                     s = madd(s, at, red[J][lo]);
-                    v = RCP ? (rprev - s) * (T(1) / ap) : (rprev - s) / ap;
+                    if constexpr (HOIST) {
+                        const int m = pcls(z);
+                        v = (rprev - s) * (m == 0 ? iao[0] : (m == 2 ? iao[2] : iao[1]));
+                    } else {
+                        v = RCP ? (rprev - s) * (T(1) / ap) : (rprev - s) / ap;
+                    }
                 }
                 bstore<T>(rs_out, v, o_own, (uint32_t)z * pb);
                 const int iz = own_ghosts ? ghost_image(z, g.n2, g.neu & kNeuB, g.neu & kNeuT) : -1;
