@@ -332,7 +332,6 @@ std::string ghost_exprs(const std::vector<mf3d_block_desc> &blk, const std::vect
 std::string refined_scope(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (!any_refined(c.nblocks, b)) return "";
     if (c.variable_coef) return "refined blocks (rdiv > 1) with variable coefficients";
-    if (c.smoother != 3) return "refined blocks (rdiv > 1) with a smoother other than red-black (mg_itrsch 3)";
     if (c.krylov != MF3D_OUTER_DEFECT) return "refined blocks (rdiv > 1) with the GCR outer iteration";
     if (c.coarse_solve == MF3D_COARSEST_DIRECT) return "refined blocks (rdiv > 1) with the direct coarsest solve";
     for (int i = 0; i < c.nblocks; ++i)

@@ -11,9 +11,12 @@ on the other axes.  The reference's coeff + mgkcyc + exchangepp (its 2:1
 branches: 8-cell averages to the coarse side, (2 phi_c + 3 phi_1 - phi_2)/4
 on the fine side, injection on the repeated coarsest grid) run in the
 Poisson form of the newsolv_mg loop for a fixed number of cycles on a
-seeded RHS.  <case>.npz holds f, rho, rmax, p and pp1 as make_golden.py
-writes them; manifest_refined.json records how each case was made
-(levels = ngrd_gl, the global level count).
+seeded RHS, with the red-black smoother (mg_itrsch 3) or, in the *_sm1 /
+*_sm2 cases, the lexicographic (1) and ADI (2) ones.  <case>.npz holds f,
+rho, rmax, p and pp1 as make_golden.py writes them; manifest_refined.json
+records how each case was made (levels = ngrd_gl, the global level count).
+Files already present are kept (rewriting one changes its zip timestamps,
+not its arrays); the manifest is written whole.
 """
 import json
 import os
@@ -31,7 +34,7 @@ from oracle.oracle import run_ref  # noqa: E402
 
 N, P = T.NEUMANN, T.PERIODIC
 WALLS = [N] * 6
-# name: (ex, ey, ez, bc, h, rdiv, levels, cycles, seed)
+# name: (ex, ey, ez, bc, h, rdiv, levels, cycles, seed[, smoother])
 CASES = {
     "ref_x_cf_walls_L4": ([8, 8], [8], [8], WALLS, [1 / 16] * 3, [1, 2], 4, 6, W.BASE_SEED + 61),
     "ref_x_fc_zper_L4": ([8, 8], [8], [8], [N, N, N, N, P, P], [1 / 16] * 3, [2, 1], 4, 6, W.BASE_SEED + 62),
@@ -41,6 +44,15 @@ CASES = {
     "ref_x_cfc_yper_L3": ([8, 12, 8], [8], [8], [N, N, P, P, N, N], [1 / 28, 1 / 8, 1 / 8], [1, 2, 1], 3, 6,
                           W.BASE_SEED + 65),
     "ref_x_r2r4_L4": ([8, 8], [8], [8], WALLS, [1 / 16] * 3, [2, 4], 4, 5, W.BASE_SEED + 66),
+    "ref_x_cf_walls_L4_sm1": ([8, 8], [8], [8], WALLS, [1 / 16] * 3, [1, 2], 4, 5, W.BASE_SEED + 67, 1),
+    "ref_x_fc_zper_L4_sm2": ([8, 8], [8], [8], [N, N, N, N, P, P], [1 / 16] * 3, [2, 1], 4, 5,
+                             W.BASE_SEED + 68, 2),
+    "ref_y_cf_xper_L3_sm1": ([8], [8, 12], [8], [P, P, N, N, N, N], [1 / 16, 1 / 20, 1 / 8], [1, 2], 3, 5,
+                             W.BASE_SEED + 69, 1),
+    "ref_z_fc_walls_L4_sm2": ([8], [8], [8, 8], WALLS, [1 / 8, 1 / 8, 1 / 16], [2, 1], 4, 5, W.BASE_SEED + 70, 2),
+    "ref_x_cfc_yper_L3_sm2": ([8, 12, 8], [8], [8], [N, N, P, P, N, N], [1 / 28, 1 / 8, 1 / 8], [1, 2, 1], 3, 5,
+                              W.BASE_SEED + 71, 2),
+    "ref_x_r2r4_L4_sm1": ([8, 8], [8], [8], WALLS, [1 / 16] * 3, [2, 4], 4, 5, W.BASE_SEED + 72, 1),
 }
 
 
@@ -52,15 +64,19 @@ def grid_of(spec):
 def main():
     manifest = {}
     for name, spec in CASES.items():
-        ex, ey, ez, bc, h, rdiv, L, cyc, seed = spec
+        ex, ey, ez, bc, h, rdiv, L, cyc, seed = spec[:9]
+        sm = spec[9] if len(spec) > 9 else 3
+        manifest[name] = dict(ex=ex, ey=ey, ez=ez, bc=bc, h=h, rdiv=rdiv, levels=L, cycles=cyc, seed=seed,
+                              tol=0.0, smoother=sm)
+        out = os.path.join(HERE, name + ".npz")
+        if os.path.exists(out):
+            continue
         g = grid_of(spec)
         f = W.block_rhs(g, seed)
-        p, rho, rmax, k, pp1 = run_ref(g, L, f, 0.0, cyc)
-        np.savez_compressed(os.path.join(HERE, name + ".npz"), f=np.concatenate([a.ravel() for a in f]),
+        p, rho, rmax, k, pp1 = run_ref(g, L, f, 0.0, cyc, smoother=sm)
+        np.savez_compressed(out, f=np.concatenate([a.ravel() for a in f]),
                             rho=rho, rmax=rmax, p=np.concatenate([a.ravel() for a in p]),
                             pp1=np.concatenate([a.ravel() for a in pp1]))
-        manifest[name] = dict(ex=ex, ey=ey, ez=ez, bc=bc, h=h, rdiv=rdiv, levels=L, cycles=cyc, seed=seed,
-                              tol=0.0)
         print(name, "cycles", k, "rho_last", rho[-1], flush=True)
     with open(os.path.join(HERE, "manifest_refined.json"), "w") as fh:
         json.dump(manifest, fh, indent=1)

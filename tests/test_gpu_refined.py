@@ -2,9 +2,10 @@
 
 - Every refined fixture of the compiled reference (rows of blocks along x,
   y or z, rdiv 1/2 and 2/4 side by side, walls or a block's own periodic
-  wrap on the other axes): FP64 pp after one V-cycle with its ghost layer
-  and p after the fixed cycles bitwise the reference's, the residual
-  history to 1e-12 (the GPU sums the norm in a different order).
+  wrap on the other axes; red-black, lexicographic and ADI smoothers):
+  FP64 pp after one V-cycle with its ghost layer and p after the fixed
+  cycles bitwise the reference's, the residual history to 1e-12 (the GPU
+  sums the norm in a different order).
 - The same cycles with the 3-D tile sweep forced on every level, and with
   the march forced: bitwise too.
 - Solves to a tolerance need a right-hand side in the operator's range:
@@ -13,8 +14,9 @@
   outside the range, where the residual stops).  f = A p_true (the oracle's
   operator, its 2:1 ghosts included) is.  Mixed precision (FP32 inner
   cycle, FP64 outer step and coarsest level) and FP64 on the fixtures'
-  grids to 1e-9: within 1e-7 of each other and 1e-6 of p_true (mean
-  removed; at a 1e-9 residual the oracle itself is 1e-9 .. 1.5e-7 from it).
+  grids to 1e-9 (red-black, and the lexicographic and ADI smoothers):
+  within 1e-7 of each other and 1e-6 of p_true (mean removed; at a 1e-9
+  residual the oracle itself is 1e-9 .. 1.5e-7 from it).
 - A larger refined row (a 64 x 32 x 32 coarse block beside a 64^3 refined
   one, 5 levels): bitwise against the oracle for 2 cycles in FP64, then
   both precisions to 1e-8 on f = A p_true.
@@ -63,13 +65,14 @@ def _solve(g, L, f, **kw):
 @pytest.mark.parametrize("name", sorted(MAN))
 def test_refined_fp64_matches_reference_fixture(name):
     g, m, a = _load(name)
+    sm = m.get("smoother", 3)
     print(f"[refined fp64] {name}", flush=True)
-    with PressureMultigrid(g, m["levels"]) as mg:
+    with PressureMultigrid(g, m["levels"], smoother=sm) as mg:
         pp = [np.zeros((b.n[2] + 2, b.n[1] + 2, b.n[0] + 2)) for b in g.blocks]
         mg.mgkcyc(a["f"], pp)
     for x, e in zip(pp, a["pp1"]):
         assert np.array_equal(x, e), np.abs(x - e).max()
-    p, res = _solve(g, m["levels"], a["f"], maxcy=m["cycles"], eps=0.0)
+    p, res = _solve(g, m["levels"], a["f"], maxcy=m["cycles"], eps=0.0, smoother=sm)
     assert res.iters == m["cycles"]
     np.testing.assert_allclose(res.hist, a["rho"], rtol=1e-12, atol=0)
     for x, e in zip(p, a["p"]):
@@ -113,13 +116,15 @@ def _in_range(g, L, seed):
     return f, pt
 
 
-@pytest.mark.parametrize("name", ["ref_x_cf_walls_L4", "ref_y_cf_xper_L3", "ref_x_r2r4_L4"])
+@pytest.mark.parametrize("name", ["ref_x_cf_walls_L4", "ref_y_cf_xper_L3", "ref_x_r2r4_L4", "ref_x_fc_zper_L4_sm2",
+                                  "ref_y_cf_xper_L3_sm1"])
 def test_refined_mixed_within_tolerance_of_fp64(name):
     g, m, _ = _load(name)
+    sm = m.get("smoother", 3)
     print(f"[refined mixed] {name}", flush=True)
     f, pt = _in_range(g, m["levels"], 9)
-    p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9)
-    pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, precision="mixed")
+    p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm)
+    pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, precision="mixed", smoother=sm)
     assert r64.status == 0 and rmx.status == 0 and rmx.hist[-1] <= 1e-9
     err, e64 = _rel(pmx, p64), _rel(p64, pt)
     print(f"  fp64 {r64.iters} / mixed {rmx.iters} cycles, mixed vs fp64 {err:.2e}, fp64 vs p_true {e64:.2e}",

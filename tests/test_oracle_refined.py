@@ -2,8 +2,9 @@
 
 - Against golden vectors of the compiled reference (make_golden_refined.py):
   rows of 2-3 blocks along x, y or z, rdiv 1/2 and 2/4 side by side,
-  walls or a block's own periodic wrap on the other axes: p and pp1
-  bitwise, residual histories to 1e-14.
+  walls or a block's own periodic wrap on the other axes, with the
+  red-black, lexicographic and ADI smoothers: p and pp1 bitwise, residual
+  histories to 1e-14.
 - The results do not depend on the 2:1 ghost rows no sender fills (the
   reference rewrites them from stale receive-buffer entries): with those
   rows poisoned with NaN the fixtures' cycles come out unchanged.
@@ -57,7 +58,7 @@ def test_oracle_reproduces_refined_fixture(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
     f = _split(g, z["f"])
     assert np.array_equal(_cat(W.block_rhs(g, m["seed"])), z["f"])
-    o = OracleMG(g.blocks, m["levels"])
+    o = OracleMG(g.blocks, m["levels"], smoother=m.get("smoother", 3))
     o.coeff()
     p, rho, rmax, k = o.solve(f, 0.0, m["cycles"])
     assert k == m["cycles"]
@@ -73,7 +74,7 @@ def test_fixture_independent_of_unfilled_rows(name):
     m = MAN[name]
     g = _grid(m)
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
-    o = OracleMG(g.blocks, m["levels"])
+    o = OracleMG(g.blocks, m["levels"], smoother=m.get("smoother", 3))
     o.coeff()
     o.set_poison(True)
     p, rho, _, _ = o.solve(_split(g, z["f"]), 0.0, m["cycles"])
