@@ -66,8 +66,14 @@ __device__ __forceinline__ int ghost_image(int v, int n, bool neu_lo, bool neu_h
     return v == m0 ? 0 : (v == mn ? n + 1 : -1);
 }
 
+// FP32 instances are held to 64 VGPRs (8 waves a SIMD: four 512-thread
+// workgroups a CU, LDS permitting), which they fit without spilling; FP64
+// ones would spill there.
+template <typename T, bool VAR, bool ZERO>
+constexpr int rbs_waves_per_eu() { return sizeof(T) == 4 && !(VAR && ZERO) ? 8 : 1; }
+
 template <typename T, bool VAR, bool ZERO, bool RCP, int TY>
-__global__ __launch_bounds__(SW_TX *TY) void k_rbs(const T *__restrict__ pin, T *__restrict__ pout,
+__global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void k_rbs(const T *__restrict__ pin, T *__restrict__ pout,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ cx,
                                                   const T *__restrict__ cy,
