@@ -678,6 +678,14 @@ void gather(mf3d_mg *mg, int l, const int *done) {
     fill_ghosts<T>(mg, L, (T *)L.phi[L.cur], done);
 }
 
+// The march's lag (planes between the two colours): 2 (one barrier a step)
+// for the 8-row tiles' FP32 sweeps, 1 (two barriers, one step fewer a
+// chunk) elsewhere -- by measurement: 54.6 -> 49.4 us for the 256^3 FP32
+// sweep, while the FP64 one gains nothing (118 -> 120 us) and the 4-row,
+// 8-plane chunks of 128^3 lose (13.0 -> 14.2 us).
+template <typename T>
+constexpr int kRbsLag8 = sizeof(T) == 4 ? 2 : 1;
+
 // The mixed form's FP32 inner cycle multiplies by 1/ap; the FP64 form
 // divides, as the reference does.
 template <typename T, bool VAR, bool ZERO>
@@ -705,11 +713,11 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
         return;
     }
     if (L.sw_ty == 8)
-        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8, kRbsLag8<T>>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
                            L.t_sw, done);
     else
-        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 4>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 4, 1>), dim3(L.n_sw), dim3(SW_TX, 4), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
                            L.t_sw, done);
 }
@@ -1757,9 +1765,10 @@ int autotune_sweeps(mf3d_mg *mg) {
         if (!choice) {
             int nzmax = 1;
             for (const BlockLevel &g : V.hb) nzmax = std::max(nzmax, g.n2);
-            // powers of two, and depths whose step count (tz + 2) is a whole
-            // number of the march's 10-step unrolled blocks
-            static const int kTz[] = {2, 4, 8, 16, 18, 28, 32, 38, 48, 58, 64};
+            // powers of two, and depths whose step count (tz + 2, or tz + 4
+            // at lag 2) is a whole number of the march's 10- (lag 1) or
+            // 6-step (lag 2) unrolled blocks
+            static const int kTz[] = {2, 4, 8, 14, 16, 18, 20, 26, 28, 32, 38, 44, 48, 50, 56, 58, 62, 64};
             std::vector<std::pair<int, std::vector<Tile>>> cand;
             size_t most = 1;
             for (int tz : kTz) {

@@ -16,29 +16,35 @@ namespace mf3d {
 //
 // A workgroup owns SW_TX x TY columns of one block and marches over the
 // planes z0..z1 (z0 odd: chunks start at 1 and have even depth).  LDS keeps
-// a ring of 5 planes of pin, each with a 2-cell halo in x and y.  At step c
+// a ring of planes of pin, each with a 2-cell halo in x and y.  At step c
 // the first colour is computed on plane c over the tile plus a 1-cell ring
-// (in place in LDS), then the second colour on plane c-1: half the threads
-// compute its cells, the other half copy the first-colour cells of that
-// plane out of LDS.  Red cells in the ring are recomputed by both
+// (in place in LDS), then the second colour on plane c-LAG: half the
+// threads compute its cells, the other half copy the first-colour cells of
+// that plane out of LDS.  Red cells in the ring are recomputed by both
 // neighbouring workgroups from the same inputs, so they are bitwise the
-// values the reference computes once.
+// values the reference computes once.  LAG 1 (5 slots) needs two barriers
+// a step (plane c+1 stored before the first colour reads it; the first
+// colour of plane c done before the second colour of plane c-1 reads it);
+// LAG 2 (6 slots) needs one, at the end of the step, since everything a
+// step reads was written in earlier steps -- at the price of one more step
+// a chunk.
 //
 // The kernel is bound by instruction issue before HBM, so the march keeps
-// scalar and address work out of the loop.  The loop is unrolled by 10
-// (the ring has 5 slots and colours alternate), which makes every LDS
-// slot, colour parity and register set compile-time.  A chunk whose step
-// count is not a multiple of 10 starts up to 8 steps early: those steps
-// load clamped planes and compute first-colour cells that no stored cell
-// reads (their slots are overwritten by whole planes before use), and
-// store nothing.  Global traffic goes through buffer resources: a
-// per-thread 32-bit offset and a per-plane scalar offset, no address
-// arithmetic per load; loads are unconditional (clamped in-bounds offsets)
-// and never masked, since values outside the block feed no update.  NR
-// register sets rotate, so the loads for step c+NR are in flight while
-// step c computes: 2 for a full sweep (four loads a step keep the queue
-// deep), 5 for the zero-start sweep, which loads only the two RHS values
-// and would otherwise wait on the stores of the steps before.  Where no
+// scalar and address work out of the loop.  The loop is unrolled by whole
+// turns of the ring (10 steps at 5 slots, 6 at 6: colours alternate, so
+// the turn is even), which makes every LDS slot, colour parity and
+// register set compile-time.  A chunk whose step count is not a multiple
+// of the turn starts up to a turn early: those steps load clamped planes
+// and compute first-colour cells that no stored cell reads (their slots
+// are overwritten by whole planes before use), and store nothing.  Global
+// traffic goes through buffer resources: a per-thread 32-bit offset and a
+// per-plane scalar offset, no address arithmetic per load; loads are
+// unconditional (clamped in-bounds offsets) and never masked, since values
+// outside the block feed no update.  NR register sets rotate, so the loads
+// for step c+NR are in flight while step c computes: 2 for a full sweep
+// (four loads a step keep the queue deep), one per slot for the
+// zero-start sweep, which loads only the two RHS values and would
+// otherwise wait on the stores of the steps before.  Where no
 // Neumann-zeroed coefficient is in reach of the tile the constant
 // coefficients and ap are hoisted.
 // ZERO: pin is identically zero (the first sweep after restriction, where
@@ -47,7 +53,6 @@ namespace mf3d {
 // precision form only; the FP64 form divides, as the reference does).
 // `bytes` is the arena length in bytes (below 2^32, checked by the host).
 constexpr int SW_TX = 64;
-constexpr int SW_UNROLL = 10;
 
 template <int J>
 using Step = std::integral_constant<int, J>;
@@ -72,7 +77,7 @@ __device__ __forceinline__ int ghost_image(int v, int n, bool neu_lo, bool neu_h
 template <typename T, bool VAR, bool ZERO>
 constexpr int rbs_waves_per_eu() { return sizeof(T) == 4 && !(VAR && ZERO) ? 8 : 1; }
 
-template <typename T, bool VAR, bool ZERO, bool RCP, int TY>
+template <typename T, bool VAR, bool ZERO, bool RCP, int TY, int LAG>
 __global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void k_rbs(const T *__restrict__ pin, T *__restrict__ pout,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ cx,
@@ -82,7 +87,10 @@ __global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void
                                                   const Tile *__restrict__ tiles,
                                                   const int *__restrict__ done) {
     if (done && *done) return;
-    constexpr int LX = SW_TX + 4, LY = TY + 4, NS = 5, NT = SW_TX * TY, LXY = LX * LY;
+    static_assert(LAG == 1 || LAG == 2, "the second colour trails the first by 1 or 2 planes");
+    constexpr int LX = SW_TX + 4, LY = TY + 4, NT = SW_TX * TY, LXY = LX * LY;
+    constexpr int NS = LAG == 2 ? 6 : 5;        // ring slots
+    constexpr int UNROLL = LAG == 2 ? 6 : 10;   // whole turns of the ring, even
     constexpr int KL = (LXY + NT - 1) / NT;
     constexpr int PS = KL * NT;            // slot stride: every thread stores KL values
     constexpr int HALF = (SW_TX + 2) / 2;  // first-colour cells per ring row
@@ -193,33 +201,34 @@ __global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void
         T pre[KL];
         T rr, rb;
     };
-    // the values step c needs: plane c+1 of pin, the RHS of this thread's red
-    // cell on plane c (parity PR) and of its second-phase cell on plane c-1
+    // the values step c needs: plane c+LAG of pin, the RHS of this thread's
+    // red cell on plane c (parity PR) and of its second-phase cell on plane
+    // c-LAG (parity index BP)
     auto load_regs = [&](Regs &R, int c, auto parc) {
-        constexpr int PR = decltype(parc)::value;
+        constexpr int PR = decltype(parc)::value, BP = LAG == 2 ? PR : 1 - PR;
         if (!ZERO) {
-            const int zc = c + 1 < zhi ? c + 1 : zhi;
+            const int zc = c + LAG < zhi ? c + LAG : zhi;
             const uint32_t so = (uint32_t)(zc < 0 ? 0 : zc) * pb;
 #pragma unroll
             for (int k = 0; k < KL; ++k) R.pre[k] = bload<T>(rs_in, lofs[k], so);
         }
         const int cr = c < 0 ? 0 : (c > g.n2 + 1 ? g.n2 + 1 : c);
         R.rr = bload<T>(rs_rhs, ro[PR], (uint32_t)cr * pb);
-        const int cb1 = c - 1 < 0 ? 0 : (c - 1 > g.n2 + 1 ? g.n2 + 1 : c - 1);
-        R.rb = bload<T>(rs_rhs, bo[1 - PR], (uint32_t)cb1 * pb);
+        const int cb = c - LAG < 0 ? 0 : (c - LAG > g.n2 + 1 ? g.n2 + 1 : c - LAG);
+        R.rb = bload<T>(rs_rhs, bo[BP], (uint32_t)cb * pb);
     };
 
     const int c0 = t.z0 - 1;  // even: chunks start at odd z0
-    int nsteps = t.z1 - t.z0 + 3;
-    nsteps += nsteps & 1;     // a last chunk of odd depth gets one idle step
-    const int pad = (SW_UNROLL - nsteps % SW_UNROLL) % SW_UNROLL;  // even
+    int nsteps = t.z1 - t.z0 + 2 + LAG;  // steps c0 .. z1+LAG
+    nsteps += nsteps & 1;
+    const int pad = (UNROLL - nsteps % UNROLL) % UNROLL;  // even
     const int cm = c0 - pad;  // first step
-    // plane z lives in slot (z - cm + 2) mod 5
-    auto slot = [&](int z) { return ((z - cm + 2) % NS + NS) % NS; };
+    // plane z lives in slot (z - cm + 1 + LAG) mod NS
+    auto slot = [&](int z) { return ((z - cm + 1 + LAG) % NS + NS) % NS; };
 
-    // prologue: planes z0-2 and z0-1
+    // prologue: planes z0-2 .. z0-2+LAG
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
+    for (int d = 0; d <= LAG; ++d) {
         const int c = t.z0 - 2 + d;
         if (c >= 0) {
             T *P = sp + slot(c) * PS;
@@ -227,8 +236,9 @@ __global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void
             for (int k = 0; k < KL; ++k) P[tid + k * NT] = ZERO ? T(0) : bload<T>(rs_in, lofs[k], (uint32_t)c * pb);
         }
     }
+    __syncthreads();
 
-    constexpr int NR = ZERO ? 5 : 2;  // divides the unroll
+    constexpr int NR = ZERO ? NS : 2;  // divides the unroll
     Regs RS[NR];
     if (ZERO) {
 #pragma unroll
@@ -238,42 +248,52 @@ __global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void
     }
     load_regs(RS[0], cm, Step<0>{});
     load_regs(RS[1], cm + 1, Step<1>{});
-    if constexpr (NR == 5) {
-        load_regs(RS[2], cm + 2, Step<0>{});
-        load_regs(RS[3], cm + 3, Step<1>{});
-        load_regs(RS[4], cm + 4, Step<0>{});
+    if constexpr (NR >= 5) {
+        load_regs(RS[2 % NR], cm + 2, Step<0>{});
+        load_regs(RS[3 % NR], cm + 3, Step<1>{});
+        load_regs(RS[4 % NR], cm + 4, Step<0>{});
     }
+    if constexpr (NR == 6) load_regs(RS[5 % NR], cm + 5, Step<1>{});
 
-    // one step on plane c with parity PR; s_*: slots of planes c-2 .. c+1
-    auto step = [&](int c, auto parc, auto setc, int s_m2, int s_m1, int s_c, int s_p1) {
-        constexpr int PR = decltype(parc)::value;
-        Regs &R = RS[decltype(setc)::value];
+    // Step J of a turn: the first colour on plane c = cm + UNROLL m + J
+    // (parity J & 1), the second on plane c - LAG.  Plane c+d sits in slot
+    // (J + 1 + LAG + d) mod NS.
+    // LAG 1: two barriers, after plane c+1 is stored and after the first
+    // colour of plane c, which the second colour of plane c-1 reads.
+    // LAG 2: one barrier, at the end: the step reads planes c-3 .. c+1, whose
+    // stores and first-colour updates all come from earlier steps, and
+    // writes plane c+2 (over plane c-4) and the first colour of plane c,
+    // neither of which it reads.
+    auto step = [&](int c, auto jc) {
+        constexpr int J = decltype(jc)::value, PR = J & 1, BP = LAG == 2 ? PR : 1 - PR;
+        constexpr auto sl = [](int d) { return ((J + 1 + LAG + d) % NS + NS) % NS; };
+        Regs &R = RS[J % NR];
         {
-            T *P = sp + s_p1 * PS;  // beyond zhi the slot holds a plane nobody reads
+            T *P = sp + sl(LAG) * PS;  // beyond zhi the slot holds a plane nobody reads
 #pragma unroll
             for (int k = 0; k < KL; ++k) P[tid + k * NT] = R.pre[k];
         }
         const T rr_cur = R.rr, rb_cur = R.rb;
         load_regs(R, c + NR, Step<(PR + NR) & 1>{});
-        __syncthreads();
+        if constexpr (LAG == 1) __syncthreads();
         if (c >= 1 && c <= g.n2 && rok[PR]) {
-            T *P = sp + s_c * PS;
+            T *P = sp + sl(0) * PS;
             const int l = rl[PR];
-            P[l] = update(P, sp + s_m1 * PS, sp + s_p1 * PS, l, rx[PR], ry, c, ro[PR], rr_cur);
+            P[l] = update(P, sp + sl(-1) * PS, sp + sl(1) * PS, l, rx[PR], ry, c, ro[PR], rr_cur);
         }
-        __syncthreads();
-        const int z = c - 1;
-        if (z >= t.z0 && z <= t.z1 && bok[1 - PR]) {
-            const T *P = sp + s_m1 * PS;
-            const int l = bl[1 - PR];
+        if constexpr (LAG == 1) __syncthreads();
+        const int z = c - LAG;
+        if (z >= t.z0 && z <= t.z1 && bok[BP]) {
+            const T *P = sp + sl(-LAG) * PS;
+            const int l = bl[BP];
             T v;
             if (blk)
-                v = update(P, sp + s_m2 * PS, sp + s_c * PS, l, bx[1 - PR], by, z, bo[1 - PR], rb_cur);
+                v = update(P, sp + sl(-LAG - 1) * PS, sp + sl(1 - LAG) * PS, l, bx[BP], by, z, bo[BP], rb_cur);
             else
                 v = P[l];
-            bstore<T>(rs_out, v, bo[1 - PR], (uint32_t)z * pb);
+            bstore<T>(rs_out, v, bo[BP], (uint32_t)z * pb);
             const int iz = own_ghosts ? ghost_image(z, g.n2, g.neu & kNeuB, g.neu & kNeuT) : -1;
-            const int ix = bix[1 - PR], X = bx[1 - PR];
+            const int ix = bix[BP], X = bx[BP];
             if (ix >= 0 || biy >= 0 || iz >= 0) {
                 auto img = [&](int xx, int yy, int zz) {
                     bstore<T>(rs_out, v, ob + (uint32_t)(xx + sy * yy) * W, (uint32_t)zz * pb);
@@ -289,25 +309,22 @@ __global__ __launch_bounds__(SW_TX *TY, (rbs_waves_per_eu<T, VAR, ZERO>())) void
                 }
             }
         }
+        if constexpr (LAG == 2) __syncthreads();
     };
-
-    // step J on plane c = cm + 10m + J keeps plane c in slot (J + 2) mod 5
-    // and uses register set J mod NR
-#define MF3D_RBS_STEP(J)                                                                                  \
-    step(c + J, Step<(J) & 1>{}, Step<(J) % NR>{}, (J) % NS, ((J) + 1) % NS, ((J) + 2) % NS, ((J) + 3) % NS)
-    for (int c = cm; c < c0 + nsteps; c += SW_UNROLL) {
-        MF3D_RBS_STEP(0);
-        MF3D_RBS_STEP(1);
-        MF3D_RBS_STEP(2);
-        MF3D_RBS_STEP(3);
-        MF3D_RBS_STEP(4);
-        MF3D_RBS_STEP(5);
-        MF3D_RBS_STEP(6);
-        MF3D_RBS_STEP(7);
-        MF3D_RBS_STEP(8);
-        MF3D_RBS_STEP(9);
+    for (int c = cm; c < c0 + nsteps; c += UNROLL) {
+        step(c + 0, Step<0>{});
+        step(c + 1, Step<1>{});
+        step(c + 2, Step<2>{});
+        step(c + 3, Step<3>{});
+        step(c + 4, Step<4>{});
+        step(c + 5, Step<5>{});
+        if constexpr (UNROLL == 10) {
+            step(c + 6, Step<6>{});
+            step(c + 7, Step<7>{});
+            step(c + 8, Step<8>{});
+            step(c + 9, Step<9>{});
+        }
     }
-#undef MF3D_RBS_STEP
 }
 
 // The zero-start sweep (the first sweep after restriction, or of a cycle

@@ -204,3 +204,25 @@ def test_cycle_shapes_bitwise(name, nu):
                     assert np.array_equal(a, b), f"{name}/{nu}/hook={hook}/{run}: max {np.max(np.abs(a - b))}"
             if hook:
                 assert mg.kernel_time()[0] >= 1
+
+
+@pytest.mark.parametrize("tz", ["2", "4", "6", "16"])
+@pytest.mark.parametrize("name", ["single64_L5", "multi_uneven_L3", "mixed_bc_multi_L3"])
+def test_mixed_march_depths_bitwise(name, tz, monkeypatch):
+    """The mixed form's march at pinned chunk depths (the finest level's
+    FP32 march trails the second colour by two planes, coarser levels by
+    one; each chunk pads its steps to whole turns of the LDS ring) against
+    the one-pass 3-D tiles: bitwise the same pressure."""
+    grid, L, _ = case(name)
+    f, _ = inputs(grid, False, seed=29)
+    got = {}
+    for tiles in ("3d", "march"):
+        monkeypatch.setenv("MF3D_SWEEP_3D", "1" if tiles == "3d" else "0")
+        if tiles == "march":
+            monkeypatch.setenv("MF3D_SWEEP_TZ", tz)
+        with _gpu(grid, L, False, None, maxcy=3, eps=0.0, precision="mixed") as mg:
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", NotConverged)
+                got[tiles] = mg.newsolv_mg(f)
+    for a, b in zip(got["3d"][0], got["march"][0]):
+        assert np.array_equal(a, b), f"{name}/tz={tz}: max {np.max(np.abs(a - b))}"
