@@ -73,7 +73,9 @@ struct Level {
     int sw_ty = 4;  // sweep tile rows of the march; 0: 3-D tiles (k_rb3d)
     // the zero-start march (k_rbz) on its own chunks: the default depth
     // (the deepest that still gives ~1024 workgroups), which suits its
-    // light steps better than the full sweep's tuned depth
+    // light steps better than the full sweep's tuned depth -- under
+    // constant coefficients; with variable ones (coefficient loads make
+    // its steps heavier) the full sweep's tuned tiles
     Tile *t_swz = nullptr;
     int n_swz = 0, swz_ty = 4;
     Region *r_full = nullptr;
@@ -679,12 +681,13 @@ void gather(mf3d_mg *mg, int l, const int *done) {
 }
 
 // The march's lag (planes between the two colours): 2 (one barrier a step)
-// for the 8-row tiles' FP32 sweeps, 1 (two barriers, one step fewer a
-// chunk) elsewhere -- by measurement: 54.6 -> 49.4 us for the 256^3 FP32
-// sweep, while the FP64 one gains nothing (118 -> 120 us) and the 4-row,
-// 8-plane chunks of 128^3 lose (13.0 -> 14.2 us).
-template <typename T>
-constexpr int kRbsLag8 = sizeof(T) == 4 ? 2 : 1;
+// for the 8-row tiles' FP32 constant-coefficient sweeps, 1 (two barriers,
+// one step fewer a chunk) elsewhere -- by measurement: 54.6 -> 49.4 us for
+// the 256^3 FP32 sweep, while the FP64 one gains nothing (118 -> 120 us),
+// the 4-row, 8-plane chunks of 128^3 lose (13.0 -> 14.2 us) and so does
+// the variable-coefficient FP32 sweep at 384^3 (468 -> 539 us).
+template <typename T, bool VAR>
+constexpr int kRbsLag8 = sizeof(T) == 4 && !VAR ? 2 : 1;
 
 // The mixed form's FP32 inner cycle multiplies by 1/ap; the FP64 form
 // divides, as the reference does.
@@ -713,7 +716,7 @@ void sweep_launch(mf3d_mg *mg, Level &L, const int *done) {
         return;
     }
     if (L.sw_ty == 8)
-        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8, kRbsLag8<T>>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
+        hipLaunchKernelGGL((k_rbs<T, VAR, ZERO, RCP, 8, kRbsLag8<T, VAR>>), dim3(L.n_sw), dim3(SW_TX, 8), 0, mg->stream, pin, pout,
                            (const T *)L.rhs, (const T *)L.cx, (const T *)L.cy, (const T *)L.cz, bytes, own, L.db,
                            L.t_sw, done);
     else
@@ -1805,6 +1808,11 @@ int autotune_sweeps(mf3d_mg *mg) {
         V.t_sw = dst;
         V.n_sw = (int)t.size();
         V.sw_ty = choice == kTz3d ? 0 : ty;
+        if (VAR && choice != kTz3d) {  // the zero-start march: the same tiles (384^3: 717 against 754 us
+            V.t_swz = dst;             // on its own default-depth chunks)
+            V.n_swz = V.n_sw;
+            V.swz_ty = ty;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
