@@ -1,6 +1,6 @@
 """CPU restatement of the reference's level-set density (TEST INFRASTRUCTURE
-ONLY: the checker for the library's mf3d_mg_heaviside entry points; never
-imported by the product).
+ONLY: what the tests compare the library's mf3d_mg_heaviside entry points
+against; never imported by the product).
 
 Fields are lists of ghosted block arrays (one ghost layer), shaped
 (z+2, y+2, x+2), as in oracle.velocity.

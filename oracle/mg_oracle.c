@@ -1,8 +1,9 @@
 /*
  * mg_oracle.c -- CPU FP64 restatement of MultiFlow3D's pressure multigrid.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
- * library and the CPU baseline timed beside it; the product never links it.
+ * TEST INFRASTRUCTURE ONLY.  The parity tests compare the HIP library
+ * against this file, and it is the CPU baseline timed beside it; the product
+ * never links it.
  *
  * It restates, in its own data layout (one ghosted array per block and level,
  * i fastest), the arithmetic and the order of operations of:

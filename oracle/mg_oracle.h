@@ -1,7 +1,7 @@
 /*
  * mg_oracle.h -- CPU FP64 restatement of MultiFlow3D's pressure multigrid
- * (newsolv_mg / mgkcyc / coeff).  TEST INFRASTRUCTURE ONLY: the parity
- * checker and the CPU baseline.  Nothing in the product library links it.
+ * (newsolv_mg / mgkcyc / coeff).  TEST INFRASTRUCTURE ONLY: what the parity
+ * tests compare against, and the CPU baseline.  Nothing in the product library links it.
  *
  * Levels are 0-based here (level 0 = finest = the reference's glevel 1).
  * Arrays handed in and out are concatenated over blocks in block order; a

@@ -2,8 +2,8 @@
 reference driver (_ref/mg_ref).
 
 TEST INFRASTRUCTURE ONLY.  Import it from tests/, __graft_entry__.smoke()
-and bench.py's CPU-baseline leg, and nowhere else: it is the checker, never
-the thing measured or shipped.
+and bench.py's CPU-baseline leg, and nowhere else: the tests compare the
+library against it; it is never the thing measured or shipped.
 """
 from __future__ import annotations
 
@@ -235,7 +235,7 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
         ctl = os.path.join(td, "ctl.txt")
         with open(ctl, "w") as fh:
             nbx, nby, nbz = grid.shape
-            # levels is ngrd_gl; the driver takes ngrid_input = ngrd_gl - log2(max rdiv)
+            # levels is ngrd_gl; ref_driver takes ngrid_input = ngrd_gl - log2(max rdiv)
             lin = levels - (max(getattr(b, "rdiv", 1) for b in grid.blocks).bit_length() - 1)
             fh.write(f"{nbx} {nby} {nbz} {lin} {int(lsm)} {maxcy} {min_cycles} {nu_pre} {nu_post}\n")
             fh.write(f"{tol!r} {smoother}\n")
@@ -267,7 +267,7 @@ def run_ref(grid, levels: int, f_blocks, tol: float, maxcy: int, min_cycles: int
     return p, rho[:k + 1].copy(), rmax[:k + 1].copy(), k, pp1
 
 
-LAST_REF_SECONDS = None  # solve time of the last run_ref, measured inside the driver
+LAST_REF_SECONDS = None  # solve time of the last run_ref, measured inside ref_driver
 
 
 def have_ref_vel() -> bool:

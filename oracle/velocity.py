@@ -1,6 +1,6 @@
 """CPU restatement of the velocity form of newsolv_mg (TEST INFRASTRUCTURE
-ONLY: the checker for the library's mf3d_mg_newsolv; never imported by the
-product).
+ONLY: what the tests compare the library's mf3d_mg_newsolv against; never
+imported by the product).
 
 Fields are lists of ghosted block arrays (one ghost layer), shaped
 (z+2, y+2, x+2); index 0 and n+1 are the ghost layer.  A face field u at
