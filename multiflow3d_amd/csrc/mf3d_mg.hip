@@ -205,6 +205,9 @@ struct mf3d_mg {
     // alone (k_rbz), 0 the full sweep from a zero potential (k_rbs);
     // MF3D_ZERO_SWEEP picks one, both bitwise the same
     int zmode = 1;
+    // levels on 3-D tiles fuse the prolongation into their first post-sweep
+    // (k_rb3d<..., PRO>); MF3D_FUSE_PROLONG=0 keeps them apart, bitwise the same
+    bool fuse_pro = true;
     // the velocity form's fields on the caller's ghost shell: G layers
     // (every block's `ghost`), per-block shell geometry, total length
     int G = 1;
@@ -901,6 +904,31 @@ void tail(mf3d_mg *mg, bool from_zero, const int *done) {
     V.cur = 1 - V.cur;
 }
 
+// The prolongation into level l fused with its first post-sweep: the level
+// sweeps on 3-D tiles, every block takes the trilinear correction (none
+// repeats the coarse grid) and the red-black smoother has a post-sweep.
+bool fused_prolong(const mf3d_mg *mg, int l) {
+    const Level &F = mg->lv[l];
+    return mg->fuse_pro && F.sw_ty == 0 && F.n_prep == 0 && F.n_oct > 0 && mg->cfg.nu_post >= 1 &&
+           mg->cfg.smoother == 3;
+}
+
+template <typename T, bool VAR>
+void prolong_sweep(mf3d_mg *mg, int l, const int *done) {
+    Level &C = mg->lv[l + 1], &F = mg->lv[l];
+    constexpr bool RCP = sizeof(T) == 4;
+    const uint32_t bytes = (uint32_t)(F.cells_gh * (long long)sizeof(T));
+    const bool tk = tk_on(mg, MF3D_K_SMOOTH, l);
+    if (tk) tk_mark(mg);
+    hipLaunchKernelGGL((k_rb3d<T, VAR, false, RCP, true>), dim3(F.n_sw), dim3(R3_NT), 0, mg->stream,
+                       (const T *)F.phi[F.cur], (T *)F.phi[1 - F.cur], (const T *)F.rhs, (const T *)F.cx,
+                       (const T *)F.cy, (const T *)F.cz, bytes, F.own_ghosts ? 1 : 0, F.db, F.t_sw, done,
+                       (const T *)C.phi[C.cur], C.db);
+    if (tk) tk_mark(mg);
+    F.cur = 1 - F.cur;
+    gather<T>(mg, l, done);
+}
+
 template <typename T, bool VAR>
 void vcycle_dev(mf3d_mg *mg, bool zero_start, const int *done) {
     const int L = mg->cfg.levels, lt = mg->lt;
@@ -909,6 +937,7 @@ void vcycle_dev(mf3d_mg *mg, bool zero_start, const int *done) {
         return;
     }
     relax<T, VAR>(mg, 0, mg->cfg.nu_pre, zero_start, done);
+    int post0 = mg->cfg.nu_post;  // level 0's post-sweeps left after a fused prolongation
     if (L > 1) {
         const int ldown = std::min(lt, L);
         for (int l = 1; l < ldown; ++l) {
@@ -924,11 +953,18 @@ void vcycle_dev(mf3d_mg *mg, bool zero_start, const int *done) {
         }
         for (int l = ldown - 1; l >= 0; --l) {
             if (l == L - 1) continue;
-            prolong_to<T>(mg, l, done);
-            if (l > 0) relax<T, VAR>(mg, l, mg->cfg.nu_post, false, done);
+            int post = mg->cfg.nu_post;
+            if (fused_prolong(mg, l)) {
+                prolong_sweep<T, VAR>(mg, l, done);
+                --post;
+            } else {
+                prolong_to<T>(mg, l, done);
+            }
+            if (l > 0) relax<T, VAR>(mg, l, post, false, done);
+            else post0 = post;
         }
     }
-    relax<T, VAR>(mg, 0, mg->cfg.nu_post, false, done);
+    relax<T, VAR>(mg, 0, post0, false, done);
 }
 
 // Outer step after the V-cycle of cycle k (from = k mod 2; k = 0: the
@@ -1448,6 +1484,14 @@ int adi_coef_build(mf3d_mg *mg) {
 
 int build(mf3d_mg *mg) {
     const mf3d_mg_cfg &c = mg->cfg;
+    {
+        const char *e = std::getenv("MF3D_FUSE_PROLONG");
+        if (e && std::strcmp(e, "0") != 0 && std::strcmp(e, "1") != 0) {
+            mg->err = "MF3D_FUSE_PROLONG must be 0 or 1";
+            return MF3D_EINVAL;
+        }
+        mg->fuse_pro = !(e && e[0] == '0');
+    }
     {
         const char *e = std::getenv("MF3D_ZERO_SWEEP");
         mg->zmode = e ? std::atoi(e) : 1;

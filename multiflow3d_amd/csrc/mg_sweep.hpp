@@ -491,12 +491,23 @@ __global__ __launch_bounds__(SW_TX *TY) void k_rbz(T *__restrict__ pout, const T
 // one k_rbs stores: same update expression, same operand order.
 constexpr int R3_TX = 32, R3_TY = 8, R3_TZ = 8, R3_NT = 1024;
 
-template <typename T, bool VAR, bool ZERO, bool RCP>
+//
+// PRO: the sweep after a prolongation, fused with it: pin is the level's
+// potential before the correction, and every tile and halo value is formed
+// as k_prolong forms it -- pin at the (Neumann-reflected) source plus the
+// trilinear sum of the eight coarse values of phic (level blc), in the same
+// operand order -- instead of being read from the corrected field, which is
+// never stored.  Saves the prolongation's launch and its pass over the level
+// (the levels with 3-D tiles are small: launch-bound).
+template <typename T, bool VAR, bool ZERO, bool RCP, bool PRO = false>
 __global__ __launch_bounds__(R3_NT) void k_rb3d(const T *__restrict__ pin, T *__restrict__ pout,
                                                 const T *__restrict__ rhs, const T *__restrict__ cx,
                                                 const T *__restrict__ cy, const T *__restrict__ cz, uint32_t bytes,
                                                 int own_ghosts, const BlockLevel *__restrict__ bls,
-                                                const Tile *__restrict__ tiles, const int *__restrict__ done) {
+                                                const Tile *__restrict__ tiles, const int *__restrict__ done,
+                                                const T *__restrict__ phic = nullptr,
+                                                const BlockLevel *__restrict__ blc = nullptr) {
+    static_assert(!(PRO && ZERO), "a prolongation is followed by a sweep from its result");
     if (done && *done) return;
     constexpr int LX = R3_TX + 4, LY = R3_TY + 4, LZ = R3_TZ + 4, LXY = LX * LY, LN = LXY * LZ;
     constexpr int KL = (LN + R3_NT - 1) / R3_NT;
@@ -529,7 +540,30 @@ __global__ __launch_bounds__(R3_NT) void k_rb3d(const T *__restrict__ pin, T *__
             const int x = xb + lx, y = yb + ly, z = zb + lz;
             const bool ok = p < LN && x >= 0 && x <= g.n0 + 1 && y >= 0 && y <= g.n1 + 1 && z >= 0 && z <= g.n2 + 1;
             v[k] = T(0);
-            if (!ZERO && ok) v[k] = bload<T>(rs_in, roff(x, y), (uint32_t)z * pb);
+            if constexpr (PRO) {
+                if (ok) {
+                    const BlockLevel gc = blc[t.b];
+                    auto refl = [](int a, int n, bool lo, bool hi) { return (a == 0 && lo) ? 1 : ((a == n + 1 && hi) ? n : a); };
+                    const int si = refl(x, g.n0, g.neu & kNeuW, g.neu & kNeuE);
+                    const int sj = refl(y, g.n1, g.neu & kNeuS, g.neu & kNeuN);
+                    const int sk = refl(z, g.n2, g.neu & kNeuB, g.neu & kNeuT);
+                    const T wi0 = (si & 1) ? T(1) : T(3), wi1 = (si & 1) ? T(3) : T(1);
+                    const T wj0 = (sj & 1) ? T(1) : T(3), wj1 = (sj & 1) ? T(3) : T(1);
+                    const T wk0 = (sk & 1) ? T(1) : T(3), wk1 = (sk & 1) ? T(3) : T(1);
+                    const long long c = gidx(gc, x >> 1, y >> 1, z >> 1);
+                    T s = (wi0 * wj0 * wk0) * phic[c];
+                    s = s + (wi0 * wj1 * wk0) * phic[c + gc.sy];
+                    s = s + (wi0 * wj0 * wk1) * phic[c + gc.sz];
+                    s = s + (wi0 * wj1 * wk1) * phic[c + gc.sy + gc.sz];
+                    s = s + (wi1 * wj0 * wk0) * phic[c + 1];
+                    s = s + (wi1 * wj1 * wk0) * phic[c + 1 + gc.sy];
+                    s = s + (wi1 * wj0 * wk1) * phic[c + 1 + gc.sz];
+                    s = s + (wi1 * wj1 * wk1) * phic[c + 1 + gc.sy + gc.sz];
+                    v[k] = bload<T>(rs_in, roff(si, sj), (uint32_t)sk * pb) + s / T(64);
+                }
+            } else if (!ZERO && ok) {
+                v[k] = bload<T>(rs_in, roff(x, y), (uint32_t)z * pb);
+            }
         }
 #pragma unroll
         for (int k = 0; k < KL; ++k)

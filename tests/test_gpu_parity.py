@@ -138,6 +138,7 @@ SWITCHES = {
     "default_depths": {"MF3D_AUTOTUNE": "0"},
     "pinned_depth": {"MF3D_SWEEP_TZ": "4"},
     "zero_start_by_full_sweep": {"MF3D_ZERO_SWEEP": "0"},
+    "prolongation_apart": {"MF3D_FUSE_PROLONG": "0"},
     "all_off": {"MF3D_TAIL": "0", "MF3D_OWN_GHOSTS": "0", "MF3D_XCD_ORDER": "0", "MF3D_AUTOTUNE": "0"},
 }
 
@@ -147,7 +148,8 @@ SWITCHES = {
 def test_layout_switches_keep_results_bitwise(name, switch, monkeypatch):
     """Each environment switch of the library changes only how the work is
     laid out (tail kernel, who writes a lone block's ghosts, tile order,
-    chunk depths, which kernel runs the zero-start sweep): FP64 stays
+    chunk depths, which kernel runs the zero-start sweep, whether a 3-D
+    tile level's first post-sweep forms the prolongation itself): FP64 stays
     bitwise the oracle's."""
     for k, v in SWITCHES[switch].items():
         monkeypatch.setenv(k, v)
@@ -226,3 +228,32 @@ def test_mixed_march_depths_bitwise(name, tz, monkeypatch):
                 got[tiles] = mg.newsolv_mg(f)
     for a, b in zip(got["3d"][0], got["march"][0]):
         assert np.array_equal(a, b), f"{name}/tz={tz}: max {np.max(np.abs(a - b))}"
+
+
+@pytest.mark.parametrize("name", ["single32_L5", "single64_L5", "multi_uneven_L3", "periodic_multi_L3",
+                                  "mixed_bc_multi_L3", "neumann_L4"])
+def test_fused_prolongation_bitwise(name, monkeypatch):
+    """The first post-sweep of a level on 3-D tiles forms the corrected
+    potential itself (prolongation fused into k_rb3d): mixed bitwise the
+    same as with k_prolong apart (MF3D_FUSE_PROLONG=0), with the 3-D tiles
+    forced on every level so that level 0 takes the fused path too."""
+    monkeypatch.setenv("MF3D_SWEEP_3D", "1")
+    grid, L, _ = case(name)
+    f, _ = inputs(grid, False, seed=31)
+    got = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("MF3D_FUSE_PROLONG", fuse)
+        with _gpu(grid, L, False, None, maxcy=4, eps=0.0, precision="mixed") as mg:
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", NotConverged)
+                got[fuse] = mg.newsolv_mg(f)
+    assert np.array_equal(got["1"][1].hist, got["0"][1].hist)
+    for a, b in zip(got["1"][0], got["0"][0]):
+        assert np.array_equal(a, b), f"{name}: fused and apart differ, max {np.max(np.abs(a - b))}"
+
+
+def test_invalid_prolong_switch_is_rejected(monkeypatch):
+    monkeypatch.setenv("MF3D_FUSE_PROLONG", "2")
+    grid, L, _ = case("multi_uneven_L3")
+    with pytest.raises(Exception, match="MF3D_FUSE_PROLONG"):
+        PressureMultigrid(grid, L)
