@@ -208,13 +208,17 @@ def test_cycle_shapes_bitwise(name, nu):
                 assert mg.kernel_time()[0] >= 1
 
 
+@pytest.mark.parametrize("zero", ["1", "0"], ids=["zero_march", "zero_full"])
 @pytest.mark.parametrize("tz", ["2", "4", "6", "16"])
 @pytest.mark.parametrize("name", ["single64_L5", "multi_uneven_L3", "mixed_bc_multi_L3"])
-def test_mixed_march_depths_bitwise(name, tz, monkeypatch):
+def test_mixed_march_depths_bitwise(name, tz, zero, monkeypatch):
     """The mixed form's march at pinned chunk depths (the finest level's
     FP32 march trails the second colour by two planes, coarser levels by
-    one; each chunk pads its steps to whole turns of the LDS ring) against
-    the one-pass 3-D tiles: bitwise the same pressure."""
+    one; each chunk pads its steps to whole turns of the LDS ring), its
+    zero-start sweep by the march over the RHS or by the full sweep on
+    zeroed planes, against the one-pass 3-D tiles: bitwise the same
+    pressure."""
+    monkeypatch.setenv("MF3D_ZERO_SWEEP", zero)
     grid, L, _ = case(name)
     f, _ = inputs(grid, False, seed=29)
     got = {}
