@@ -337,7 +337,6 @@ std::string ghost_exprs(const std::vector<mf3d_block_desc> &blk, const std::vect
 std::string refined_scope(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (!any_refined(c.nblocks, b)) return "";
     if (c.variable_coef) return "refined blocks (rdiv > 1) with variable coefficients";
-    if (c.krylov != MF3D_OUTER_DEFECT) return "refined blocks (rdiv > 1) with the GCR outer iteration";
     if (c.coarse_solve == MF3D_COARSEST_DIRECT) return "refined blocks (rdiv > 1) with the direct coarsest solve";
     for (int i = 0; i < c.nblocks; ++i)
         for (int f = 0; f < 6; ++f) {
@@ -990,11 +989,23 @@ void outer_step(mf3d_mg *mg, int from, const int *done, bool first = false) {
             hipLaunchKernelGGL(k_gcr_scalar<false>, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts,
                                mg->st);
         }
-        if (tk) tk_mark(mg);
-        hipLaunchKernelGGL((k_krylov_lds<0, double, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0, mg->stream,
-                           mg->pbuf[from], mg->kd[1 - from], mg->pbuf[1 - from], mg->f_g, nullptr, mg->kr,
-                           (T *)L0.rhs, cx, cy, cz, mg->st, L0.db, mg->t_out, mg->partials, done);
-        if (tk) tk_mark(mg);
+        if (mg->refined) {  // p_new, its exchange (2:1 ghosts), then r = f - A p_new
+            const unsigned gb = (unsigned)std::min<long long>(16384, (L0.cells_gh + 255) / 256);
+            hipLaunchKernelGGL(k_paxpy, dim3(gb), dim3(256), 0, mg->stream, mg->pbuf[1 - from], mg->pbuf[from],
+                               mg->kd[1 - from], mg->st, L0.cells_gh, done);
+            fill_ghosts<double>(mg, L0, mg->pbuf[1 - from], done);
+            if (tk) tk_mark(mg);
+            hipLaunchKernelGGL((k_krylov_lds<3, double, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0,
+                               mg->stream, mg->pbuf[1 - from], nullptr, nullptr, mg->f_g, nullptr, mg->kr,
+                               (T *)L0.rhs, cx, cy, cz, mg->st, L0.db, mg->t_out, mg->partials, done);
+            if (tk) tk_mark(mg);
+        } else {
+            if (tk) tk_mark(mg);
+            hipLaunchKernelGGL((k_krylov_lds<0, double, T, VAR, 8>), dim3(mg->n_out), dim3(SW_TX, 8), 0,
+                               mg->stream, mg->pbuf[from], mg->kd[1 - from], mg->pbuf[1 - from], mg->f_g, nullptr,
+                               mg->kr, (T *)L0.rhs, cx, cy, cz, mg->st, L0.db, mg->t_out, mg->partials, done);
+            if (tk) tk_mark(mg);
+        }
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, mg->stream, mg->partials, mg->nparts, mg->st,
                            mg->d_hist, mg->d_rmax, mg->cfg.min_cycles);
         return;

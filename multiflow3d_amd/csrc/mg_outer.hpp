@@ -175,6 +175,8 @@ __global__ __launch_bounds__(256) void k_padd(double *__restrict__ pnew, const d
 //   MODE 1:  v = z - beta d_prev -> d (ghosts included), w = A v on interior
 //            cells; partials (w . w, r . w)
 //   MODE 2:  v = z; partials (A v . w_prev, 0)
+//   MODE 3:  v = p_new as given (a refined grid's, its 2:1 ghosts from the
+//            exchange after k_paxpy); r = f - A v and partials as MODE 0
 template <int MODE, typename TA, typename TR, bool VAR, int TY>
 __global__ __launch_bounds__(SW_TX *TY) void k_krylov_lds(
     const TA *__restrict__ fa, const double *__restrict__ fb, double *__restrict__ vout,
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(SW_TX *TY) void k_krylov_lds(
         for (int k = 0; k < KL; ++k) {
             if (tid + k * NT >= LXY) continue;
             double v;
-            if (MODE == 2) {
+            if (MODE == 2 || MODE == 3) {
                 const double a = (double)fa[lsrc[k] + zo];
                 v = lok[k] ? a : 0.0;
             } else {
@@ -267,10 +269,22 @@ __global__ __launch_bounds__(SW_TX *TY) void k_krylov_lds(
         }
         __syncthreads();
     }
-    if (MODE == 0)
+    if (MODE == 0 || MODE == 3)
         block_sum_max<SW_TX * TY>(s1, s2, partials + 2 * (long long)blockIdx.x);
     else
         block_sum2<SW_TX * TY>(s1, s2, partials + 2 * (long long)blockIdx.x);
+}
+
+// p_new = p_old + alpha d on every cell of the level-0 arena, alpha from
+// the solve state (a refined grid's GCR step; its ghosts are then refilled
+// from p_new by the exchange).
+__global__ __launch_bounds__(256) void k_paxpy(double *__restrict__ pnew, const double *__restrict__ pold,
+                                               const double *__restrict__ d, const SolveState *__restrict__ st,
+                                               long long n, const int *__restrict__ done) {
+    if (done && *done) return;
+    const double a = st->alpha;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        pnew[i] = pold[i] + a * d[i];
 }
 
 // Fixed-order reduction of the partials into the GCR scalars:

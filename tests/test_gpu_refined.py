@@ -20,6 +20,8 @@
 - A larger refined row (a 64 x 32 x 32 coarse block beside a 64^3 refined
   one, 5 levels): bitwise against the oracle for 2 cycles in FP64, then
   both precisions to 1e-8 on f = A p_true.
+- GCR(1) around the V-cycle: FP64 to 1e-12 of the oracle's GCR, and both
+  forms converge.
 """
 import json
 import os
@@ -154,3 +156,29 @@ def test_refined_larger_row():
     print(f"  fp64 {r64.iters} / mixed {rmx.iters} cycles, mixed vs fp64 {err:.2e}, fp64 vs p_true {e64:.2e}",
           flush=True)
     assert err < 1e-6 and e64 < 1e-4
+
+
+@pytest.mark.parametrize("name", ["ref_x_cf_walls_L4", "ref_x_fc_zper_L4", "ref_x_cfc_yper_L3", "ref_x_r2r4_L4"])
+def test_refined_gcr_matches_oracle(name):
+    """GCR(1) around the V-cycle (krylov) on refined grids: its p update
+    takes the 2:1 exchange before the true residual (k_paxpy, the ghost
+    table, then the residual pass), as the oracle's orc_solve_gcr does.
+    FP64 after a fixed number of steps: the residual history to 1e-12 and
+    the pressure to 1e-12 of the oracle's (not bitwise: the step lengths
+    are ratios of dot products the GPU sums in another order); and both
+    forms converge to 1e-9 on f = A p_true, mixed within 1e-7 of FP64."""
+    g, m, a = _load(name)
+    print(f"[refined gcr] {name}", flush=True)
+    o = OracleMG(g.blocks, m["levels"])
+    o.coeff()
+    p_o, rho_o, _, k_o = o.solve(a["f"], 0.0, 5, gcr=True)
+    o.close()
+    p, res = _solve(g, m["levels"], a["f"], maxcy=5, eps=0.0, krylov=True)
+    assert res.iters == k_o == 5
+    np.testing.assert_allclose(res.hist, rho_o, rtol=1e-12, atol=0)
+    assert _rel(p, p_o) <= 1e-12
+    f, pt = _in_range(g, m["levels"], 13)
+    p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, krylov=True)
+    pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, krylov=True, precision="mixed")
+    assert r64.status == 0 and rmx.status == 0
+    assert _rel(pmx, p64) < 1e-7 and _rel(p64, pt) < 1e-6
