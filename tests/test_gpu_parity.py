@@ -256,8 +256,9 @@ def test_fused_prolongation_bitwise(name, monkeypatch):
         assert np.array_equal(a, b), f"{name}: fused and apart differ, max {np.max(np.abs(a - b))}"
 
 
-def test_invalid_prolong_switch_is_rejected(monkeypatch):
-    monkeypatch.setenv("MF3D_FUSE_PROLONG", "2")
+@pytest.mark.parametrize("var,val", [("MF3D_FUSE_PROLONG", "2"), ("MF3D_ZERO_SWEEP", "2")])
+def test_invalid_switch_is_rejected(var, val, monkeypatch):
+    monkeypatch.setenv(var, val)
     grid, L, _ = case("multi_uneven_L3")
-    with pytest.raises(Exception, match="MF3D_FUSE_PROLONG"):
+    with pytest.raises(Exception, match=var):
         PressureMultigrid(grid, L)
