@@ -200,6 +200,9 @@ struct mf3d_mg {
     bool c64 = false;
     double *c64_phi[2] = {nullptr, nullptr}, *c64_rhs = nullptr;
     CCell *d_ccell = nullptr;
+    // refined block grids: the coarsest operator, assembled on the host
+    // (constant coefficients only), row-major nc x nc
+    std::vector<double> cmat;
     double *d_cinv = nullptr, *d_ccol = nullptr;
     // the zero-start sweep of the march levels: 1 a march over the RHS
     // alone (k_rbz), 0 the full sweep from a zero potential (k_rbs);
@@ -333,11 +336,11 @@ std::string ghost_exprs(const std::vector<mf3d_block_desc> &blk, const std::vect
 // corner alone: the reference's corner and edge messages have 2:1 branches
 // of their own, exchangepp.for:433-1093), a periodic axis only as a lone
 // block's wrap onto itself (exchbc_mgpp has no 2:1 branch), constant
-// coefficients, red-black smoothing, the defect-correction Poisson form.
+// coefficients (the reference's LSM coeff indexes its level arrays past
+// their bounds for blocks with fewer levels than the grid).
 std::string refined_scope(const mf3d_mg_cfg &c, const mf3d_block_desc *b) {
     if (!any_refined(c.nblocks, b)) return "";
     if (c.variable_coef) return "refined blocks (rdiv > 1) with variable coefficients";
-    if (c.coarse_solve == MF3D_COARSEST_DIRECT) return "refined blocks (rdiv > 1) with the direct coarsest solve";
     for (int i = 0; i < c.nblocks; ++i)
         for (int f = 0; f < 6; ++f) {
             if (b[i].bc[f] == 5 && (b[i].per[f] >= 0 || b[i].nbr[f] >= 0 || b[i].nbr[f ^ 1] >= 0))
@@ -1331,7 +1334,10 @@ int decide_coarse(mf3d_mg *mg) {
 // region table to the cell the exchange copies into it), and one pinned
 // cell per group of blocks connected through neighbours or wraps (the
 // operator's constants); then the matrix buffers.
+int coarse_setup_refined(mf3d_mg *mg);
+
 int coarse_setup(mf3d_mg *mg) {
+    if (mg->refined) return coarse_setup_refined(mg);
     const int L = mg->cfg.levels, nb = mg->cfg.nblocks;
     const Level &C = mg->lv[L - 1];
     int mx = 0;
@@ -1394,17 +1400,134 @@ int coarse_setup(mf3d_mg *mg) {
     return 0;
 }
 
+// Refined block grids (constant coefficients): the coarsest operator
+// assembled on the host.  A ghost neighbour is the expression of interior
+// cells the level's exchange writes into it (its ghost table): a copy, the
+// 8-cell average (0.125 each) or the 2:1 extrapolation ((2 s0 + 3 s1) -
+// s2) / 4 (weights 0.5, 0.75, -0.25), so its coefficient spreads over those
+// cells with those weights.  Pinned cells as in coarse_setup: the last cell
+// of each group of cells connected through the operator.
+int coarse_setup_refined(mf3d_mg *mg) {
+    const int L = mg->cfg.levels, nb = mg->cfg.nblocks;
+    const Level &C = mg->lv[L - 1];
+    std::vector<GhostExpr> gx;
+    const std::string why = ghost_exprs(mg->blk, mg->own, L, L - 1, &gx);
+    if (!why.empty()) {
+        mg->err = why;
+        return MF3D_ENOTSUP;
+    }
+    const long long arena = C.cells_gh;
+    std::vector<int> expr_of((size_t)arena, -1), cell_of((size_t)arena, -1);
+    for (size_t e = 0; e < gx.size(); ++e) expr_of[(size_t)gx[e].dst] = (int)e;
+    const int nc = mg->nc;
+    std::vector<CCell> cells(nc);
+    {
+        int q = 0;
+        for (int b = 0; b < nb; ++b) {
+            const BlockLevel &g = C.hb[b];
+            for (int k = 1; k <= g.n2; ++k)
+                for (int j = 1; j <= g.n1; ++j)
+                    for (int i = 1; i <= g.n0; ++i, ++q) {
+                        cell_of[(size_t)(g.off + i + g.sy * j + g.sz * k)] = q;
+                        CCell &e = cells[q];
+                        e.b = b; e.i = i; e.j = j; e.k = k; e.pin = 0;
+                        for (int d = 0; d < 6; ++d) e.nb[d] = q;
+                    }
+        }
+        if (q != nc) {
+            mg->err = "coarsest level: cell count (internal)";
+            return MF3D_EDEVICE;
+        }
+    }
+    struct Entry { int r, c; double w; };
+    std::vector<Entry> ent;
+    std::vector<int> parent(nc);
+    for (int q = 0; q < nc; ++q) parent[q] = q;
+    std::function<int(int)> root = [&](int q) { return parent[q] == q ? q : parent[q] = root(parent[q]); };
+    for (int r = 0; r < nc; ++r) {
+        const CCell &e = cells[r];
+        const BlockLevel &g = C.hb[e.b];
+        // coefs' constant form in FP64
+        const double a[6] = {(e.i == 1 && (g.neu & kNeuW)) ? 0.0 : g.c0, (e.i == g.n0 && (g.neu & kNeuE)) ? 0.0 : g.c0,
+                             (e.j == 1 && (g.neu & kNeuS)) ? 0.0 : g.c1, (e.j == g.n1 && (g.neu & kNeuN)) ? 0.0 : g.c1,
+                             (e.k == 1 && (g.neu & kNeuB)) ? 0.0 : g.c2, (e.k == g.n2 && (g.neu & kNeuT)) ? 0.0 : g.c2};
+        double s = a[0] + a[1];
+        s = s + a[2];
+        s = s + a[3];
+        s = s + a[4];
+        s = s + a[5];
+        ent.push_back({r, r, -s});
+        const long long q = g.off + e.i + g.sy * e.j + g.sz * e.k;
+        const long long nq[6] = {q - 1, q + 1, q - g.sy, q + g.sy, q - g.sz, q + g.sz};
+        for (int d = 0; d < 6; ++d) {
+            if (a[d] == 0.0) continue;
+            auto add = [&](long long o, double w) -> bool {
+                const int c = cell_of[(size_t)o];
+                if (c < 0) return false;
+                ent.push_back({r, c, w});
+                const int ra = root(r), rb = root(c);
+                if (ra != rb) parent[ra] = rb;
+                return true;
+            };
+            bool ok;
+            if (cell_of[(size_t)nq[d]] >= 0) {
+                ok = add(nq[d], a[d]);
+            } else {
+                const int x = expr_of[(size_t)nq[d]];
+                if (x < 0) {
+                    ok = false;
+                } else {
+                    const GhostExpr &ge = gx[(size_t)x];
+                    if (ge.kind == 0) {
+                        ok = add(ge.src[0], a[d]);
+                    } else if (ge.kind == 1) {
+                        ok = true;
+                        for (int m = 0; m < 8; ++m) ok = add(ge.src[m], 0.125 * a[d]) && ok;
+                    } else {
+                        ok = add(ge.src[0], 0.5 * a[d]) && add(ge.src[1], 0.75 * a[d]) && add(ge.src[2], -0.25 * a[d]);
+                    }
+                }
+            }
+            if (!ok) {
+                mg->err = "coarsest level: a ghost with no interior expression (internal)";
+                return MF3D_EDEVICE;
+            }
+        }
+    }
+    std::vector<int> last(nc, -1);
+    for (int q = 0; q < nc; ++q) last[root(q)] = q;
+    for (int q = 0; q < nc; ++q)
+        if (last[q] >= 0) cells[last[q]].pin = 1;
+    mg->cmat.assign((size_t)nc * nc, 0.0);
+    for (int q = 0; q < nc; ++q)
+        if (cells[q].pin) mg->cmat[(size_t)q * nc + q] = 1.0;
+    for (const Entry &x : ent)
+        if (!cells[x.r].pin && !cells[x.c].pin) mg->cmat[(size_t)x.r * nc + x.c] += x.w;
+    int rc;
+    if ((rc = upload(mg, &mg->d_ccell, cells))) return rc;
+    if ((rc = dev_alloc(mg, (void **)&mg->d_cinv, sizeof(double) * (size_t)nc * nc)) ||
+        (rc = dev_alloc(mg, (void **)&mg->d_ccol, sizeof(double) * (size_t)nc)))
+        return rc;
+    return 0;
+}
+
 // Assemble the coarsest operator from the current coefficients (FP64: the
-// constant 1/h^2 or the FP64 face coefficients) and invert it in place.
+// constant 1/h^2 or the FP64 face coefficients; on refined grids the
+// host-assembled matrix) and invert it in place.
 template <bool VAR>
 int coarse_factor(mf3d_mg *mg) {
     if (!mg->cdirect) return 0;
     const Level &C = mg->lv[mg->cfg.levels - 1];
     const int nc = mg->nc;
-    fill<double>(mg, mg->d_cinv, (long long)nc * nc);
-    hipLaunchKernelGGL(k_coarse_assemble<VAR>, dim3((nc + 255) / 256), dim3(256), 0, mg->stream, mg->d_cinv, nc,
-                       (const CCell *)mg->d_ccell, C.db, (const double *)C.dcx, (const double *)C.dcy,
-                       (const double *)C.dcz);
+    if (mg->refined) {
+        HIPCHK(hipMemcpyAsync(mg->d_cinv, mg->cmat.data(), sizeof(double) * (size_t)nc * nc, hipMemcpyHostToDevice,
+                              mg->stream));
+    } else {
+        fill<double>(mg, mg->d_cinv, (long long)nc * nc);
+        hipLaunchKernelGGL(k_coarse_assemble<VAR>, dim3((nc + 255) / 256), dim3(256), 0, mg->stream, mg->d_cinv,
+                           nc, (const CCell *)mg->d_ccell, C.db, (const double *)C.dcx, (const double *)C.dcy,
+                           (const double *)C.dcz);
+    }
     if (nc <= 512) {
         hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, mg->stream, mg->d_cinv, nc, mg->d_ccol);
     } else {

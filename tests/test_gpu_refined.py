@@ -22,6 +22,8 @@
   both precisions to 1e-8 on f = A p_true.
 - GCR(1) around the V-cycle: FP64 to 1e-12 of the oracle's GCR, and both
   forms converge.
+- The exact coarsest solve: both forms converge, in no more cycles than
+  with the coarsest level's sweeps.
 """
 import json
 import os
@@ -181,4 +183,24 @@ def test_refined_gcr_matches_oracle(name):
     p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, krylov=True)
     pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, krylov=True, precision="mixed")
     assert r64.status == 0 and rmx.status == 0
+    assert _rel(pmx, p64) < 1e-7 and _rel(p64, pt) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["ref_x_cf_walls_L4", "ref_y_cf_xper_L3", "ref_x_r2r4_L4", "ref_x_fc_zper_L4_sm2"])
+def test_refined_direct_coarsest_solve(name):
+    """The exact FP64 coarsest solve on refined grids (the operator
+    assembled with each ghost neighbour spread over the interior cells its
+    2:1 expression reads): on f = A p_true both forms converge to 1e-9,
+    within 1e-7 of each other and 1e-6 of p_true, in no more cycles than
+    the coarsest level's sweeps take."""
+    g, m, _ = _load(name)
+    sm = m.get("smoother", 3)
+    print(f"[refined direct] {name}", flush=True)
+    f, pt = _in_range(g, m["levels"], 17)
+    p_s, r_s = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm)
+    p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm, coarse_solve="direct")
+    pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm, coarse_solve="direct", precision="mixed")
+    print(f"  cycles: sweeps {r_s.iters}, direct fp64 {r64.iters}, mixed {rmx.iters}", flush=True)
+    assert r_s.status == 0 and r64.status == 0 and rmx.status == 0
+    assert r64.iters <= r_s.iters
     assert _rel(pmx, p64) < 1e-7 and _rel(p64, pt) < 1e-6

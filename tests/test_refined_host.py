@@ -107,8 +107,7 @@ def test_out_of_scope_refined_sets_refused(spec):
         PressureMultigrid(g, 3)
 
 
-@pytest.mark.parametrize("kw", [dict(variable_coef=True),
-                                dict(coarse_solve="direct")])
+@pytest.mark.parametrize("kw", [dict(variable_coef=True)])
 def test_refined_options_refused(kw):
     g = T.refined_grid([8, 8], [8], [8], [N] * 6, [1 / 16] * 3, [1, 2])
     with pytest.raises(MGError, match="refined"):
