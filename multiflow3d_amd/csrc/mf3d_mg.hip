@@ -201,7 +201,8 @@ struct mf3d_mg {
     double *c64_phi[2] = {nullptr, nullptr}, *c64_rhs = nullptr;
     CCell *d_ccell = nullptr;
     // refined block grids: the coarsest operator, assembled on the host
-    // (constant coefficients only), row-major nc x nc
+    // (constant coefficients only, so factored once, at creation; released
+    // after the inversion), row-major nc x nc
     std::vector<double> cmat;
     double *d_cinv = nullptr, *d_ccol = nullptr;
     // the zero-start sweep of the march levels: 1 a march over the RHS
@@ -1520,6 +1521,10 @@ int coarse_factor(mf3d_mg *mg) {
     const Level &C = mg->lv[mg->cfg.levels - 1];
     const int nc = mg->nc;
     if (mg->refined) {
+        if (mg->cmat.size() != (size_t)nc * nc) {
+            mg->err = "refined grid: the coarsest operator is factored once, at creation";
+            return MF3D_EINVAL;
+        }
         HIPCHK(hipMemcpyAsync(mg->d_cinv, mg->cmat.data(), sizeof(double) * (size_t)nc * nc, hipMemcpyHostToDevice,
                               mg->stream));
     } else {
@@ -1545,6 +1550,7 @@ int coarse_factor(mf3d_mg *mg) {
                        (const CCell *)mg->d_ccell);
     HIPCHK(hipStreamSynchronize(mg->stream));
     HIPCHK(hipGetLastError());
+    if (mg->refined) std::vector<double>().swap(mg->cmat);  // the copy has completed
     return 0;
 }
 
