@@ -192,7 +192,7 @@ def test_refined_direct_coarsest_solve(name):
     assembled with each ghost neighbour spread over the interior cells its
     2:1 expression reads): on f = A p_true both forms converge to 1e-9,
     within 1e-7 of each other and 1e-6 of p_true, in no more cycles than
-    the coarsest level's sweeps take."""
+    the coarsest level's sweeps take; so does the mixed form under GCR."""
     g, m, _ = _load(name)
     sm = m.get("smoother", 3)
     print(f"[refined direct] {name}", flush=True)
@@ -200,7 +200,10 @@ def test_refined_direct_coarsest_solve(name):
     p_s, r_s = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm)
     p64, r64 = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm, coarse_solve="direct")
     pmx, rmx = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm, coarse_solve="direct", precision="mixed")
-    print(f"  cycles: sweeps {r_s.iters}, direct fp64 {r64.iters}, mixed {rmx.iters}", flush=True)
-    assert r_s.status == 0 and r64.status == 0 and rmx.status == 0
+    pkr, rkr = _solve(g, m["levels"], f, maxcy=80, eps=1e-9, smoother=sm, coarse_solve="direct", krylov=True,
+                      precision="mixed")
+    print(f"  cycles: sweeps {r_s.iters}, direct fp64 {r64.iters}, mixed {rmx.iters}, mixed GCR {rkr.iters}",
+          flush=True)
+    assert r_s.status == 0 and r64.status == 0 and rmx.status == 0 and rkr.status == 0
     assert r64.iters <= r_s.iters
-    assert _rel(pmx, p64) < 1e-7 and _rel(p64, pt) < 1e-6
+    assert _rel(pmx, p64) < 1e-7 and _rel(p64, pt) < 1e-6 and _rel(pkr, p64) < 1e-7
